@@ -1,0 +1,155 @@
+/*
+ * CPU oracle for the InsPIRe server respond path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a plain-C restatement of the reference
+ * algorithm (the Rust crate raven-inspire / inspire-rs), used by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker. The
+ * product (the HIP library under inspire-rs_amd/) never links, loads or calls
+ * anything here.
+ *
+ * Every function cites the reference file:line it restates. Arithmetic is
+ * exact over Z_q with q = 2^60 - 2^14 + 1 and follows the reference's
+ * Montgomery (R = 2^64) conventions, so NTT-domain words match the reference's
+ * words too, not only the coefficient-form outputs.
+ *
+ * Pinning (which check holds each function; see tests/test_oracle_*.py):
+ *   chacha / seed_from_u64 / gen_range / gaussian -> reference Gaussian stream
+ *       hashes (src/math/gaussian.rs:334-351), client_packing_key_kat.txt.
+ *   ntt / mont_mul -> O(n^2) negacyclic convolution brute force, round trips,
+ *       Montgomery == plain (a*b*2^-64 mod q by u128 division).
+ *   automorph tables -> invariant NTT(tau_t f) = NTT(f)[table], and the
+ *       closed form of the bit-reversed psi ladder.
+ *   whole respond (no store, exact gadget) -> respond_byte_identity_d256.bin,
+ *       FNV-1a 0x796a80ca60d31a52 (tests/respond_byte_identity_kat.rs:108-145).
+ *   store path -> equality with the no-store path (an invariant the reference
+ *       tests too, tests/shard_packing_store.rs:163-208).
+ *   approximate gadget -> decode correctness only: "parity unpinned" at the
+ *       byte level (no committed reference vector exists).
+ *   mod switch -> u128 closed form brute force.
+ */
+#ifndef INSPIRE_ORACLE_H
+#define INSPIRE_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_Q 1152921504606830593ULL /* 2^60 - 2^14 + 1, src/math/mod_q.rs:7 */
+
+/* ---------------- ChaCha20Rng (rand_chacha 0.3.1 semantics) ---------------- */
+typedef struct {
+    uint32_t key[8];
+    uint64_t counter;   /* next block counter */
+    uint64_t stream;
+    uint32_t buf[64];   /* 4 blocks */
+    int index;          /* 64 = empty */
+} orc_chacha;
+
+void orc_chacha_from_seed(orc_chacha *r, const uint8_t seed[32]);
+void orc_chacha_seed_from_u64(orc_chacha *r, uint64_t state);
+void orc_chacha_set_stream(orc_chacha *r, uint64_t stream);
+uint32_t orc_chacha_next_u32(orc_chacha *r);
+uint64_t orc_chacha_next_u64(orc_chacha *r);
+void orc_chacha_fill_bytes(orc_chacha *r, uint8_t *out, size_t len);
+uint64_t orc_gen_range_u64(orc_chacha *r, uint64_t low, uint64_t high_excl);
+int64_t orc_gen_range_i64_incl(orc_chacha *r, int64_t low, int64_t high);
+double orc_gen_f64_unit(orc_chacha *r);
+/* raw keystream words (for tests of block layout) */
+void orc_chacha_block(const uint32_t key[8], uint64_t counter, uint64_t stream, uint32_t out[16]);
+
+/* ---------------- Gaussian sampler (src/math/gaussian.rs) ---------------- */
+typedef struct {
+    double sigma;
+    int64_t tailcut;
+    uint64_t *accept_bits; /* 2*tailcut+1 */
+    orc_chacha rng;
+} orc_gauss;
+
+void orc_gauss_with_seed(orc_gauss *g, double sigma, uint64_t seed);
+void orc_gauss_from_seed(orc_gauss *g, double sigma, const uint8_t seed[32]);
+void orc_gauss_free(orc_gauss *g);
+int64_t orc_gauss_sample(orc_gauss *g);
+uint64_t fnv1a(const uint8_t *bytes, size_t len);
+/* hash of n samples (i64 LE bytes), the reference stream-hash test */
+uint64_t orc_gauss_stream_hash(double sigma, uint64_t seed, size_t n);
+uint64_t orc_gauss_centered_hash_seed7(size_t n, uint64_t q);
+
+/* ---------------- NTT (src/math/ntt.rs) ---------------- */
+typedef struct {
+    size_t n;
+    uint64_t q, q_inv_neg, r2, psi, psi_inv, n_inv_mont;
+    uint64_t *psi_pow;     /* Montgomery, bit-reversed ladder */
+    uint64_t *psi_inv_pow;
+} orc_ntt;
+
+int orc_ntt_init(orc_ntt *c, size_t n, uint64_t q);
+void orc_ntt_free(orc_ntt *c);
+uint64_t orc_mont_mul(const orc_ntt *c, uint64_t a, uint64_t b);
+uint64_t orc_to_mont(const orc_ntt *c, uint64_t a);
+void orc_ntt_forward(const orc_ntt *c, uint64_t *a);   /* standard -> Montgomery NTT */
+void orc_ntt_inverse(const orc_ntt *c, uint64_t *a);   /* Montgomery NTT -> standard */
+void orc_negacyclic_mul_naive(size_t n, uint64_t q, const uint64_t *a, const uint64_t *b, uint64_t *out);
+
+/* ---------------- InspiRING packing (src/inspiring/inspiring2.rs) ---------------- */
+typedef struct {
+    size_t n, gamma;
+    uint64_t q;
+    size_t generator;
+    size_t *gen_pows;        /* n */
+    uint64_t mod_inv_gamma;
+    uint64_t gadget_base;
+    size_t gadget_len, dropped; /* key_rows = gadget_len - dropped */
+    uint64_t *mono_ntt;      /* n x n, NTT(X^j) */
+    uint64_t *neg_mono_ntt;  /* n x n, NTT(-X^j) */
+    uint32_t *tables;        /* gamma x n: table of tau_{g^i}, i < gamma */
+    orc_ntt ntt;
+} orc_pack;
+
+int orc_pack_init(orc_pack *p, size_t n, size_t gamma, size_t gadget_len, size_t dropped);
+void orc_pack_free(orc_pack *p);
+/* full automorph table for odd t (derive_automorph_tables, inspiring2.rs:330-379) */
+int orc_automorph_table(const orc_pack *p, size_t t, uint32_t *out);
+
+/* w_mask rows as the offline keys hold them (kept_mask_rows + expand_mask, inspiring2.rs:839-887),
+ * coefficient form, key_rows x n */
+void orc_expand_w_mask(const orc_pack *p, const uint8_t w_seed[32], uint64_t *w_mask);
+
+/* rotation_sums (inspiring2.rs:964-1004): inputs_ntt count x n, out gamma x n (NTT form) */
+void orc_rotation_sums(const orc_pack *p, const uint64_t *inputs_ntt, size_t count, uint64_t *out);
+
+/* ShardPackingStore::build step-1 sums S_i (shard_store.rs:86-156), columns gamma x n coefficient
+ * form -> gamma x n NTT form */
+void orc_store_build(const orc_pack *p, const uint64_t *columns, uint64_t *sums_ntt);
+
+/* fold_stripe coefficient-0 part (shard_store.rs:197-265, 378-427): b_poly[j] = coeff0(b * h_j) */
+void orc_fold_b(const orc_pack *p, const uint64_t *columns, const uint64_t *b, uint64_t *b_poly);
+
+/* The full respond (pack_shard, respond.rs:770-863) given expanded inputs.
+ *   a, b          : the query's RLWE row (coefficient form)
+ *   columns       : gamma x n column polynomials
+ *   y_body        : key_rows x n client key body (coefficient form)
+ *   sums_ntt      : NULL for the no-store path (plaintext multiply + packing_offline),
+ *                   else the store's S_i (the store path)
+ * outputs a_hat (n) and b (n, full polynomial). */
+int orc_respond(const orc_pack *p, const uint8_t w_seed[32], const uint64_t *a, const uint64_t *b,
+                const uint64_t *columns, const uint64_t *y_body, const uint64_t *sums_ntt,
+                uint64_t *a_hat, uint64_t *b_out);
+
+/* mod_switch_poly (src/pir/mod_switch.rs:148-175) */
+void orc_mod_switch(const uint64_t *in, size_t count, uint64_t q_old, uint64_t q_new, uint64_t *out);
+
+/* ServerResponse bincode (respond.rs:111-176, poly.rs:166-225, 286-310). Returns length;
+ * out may be NULL to query the length. */
+size_t orc_encode_response(size_t n, uint64_t modulus, const uint64_t *a_hat, const uint64_t *b,
+                           size_t retained, uint8_t *out);
+
+/* query a from its seed (Poly::from_seed_moduli, poly.rs:682 -> :657) */
+void orc_expand_seed(const uint8_t seed[32], size_t n, uint64_t q, uint64_t *a);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
