@@ -1,0 +1,139 @@
+/*
+ * inspire_gpu.h - C ABI of the MI355X (gfx950) InsPIRe respond path.
+ *
+ * Plain pointers and sizes only. Every entry point returns INSPIRE_OK (0) or an
+ * error code and, on error, writes a NUL-terminated message into err[0..err_cap)
+ * (the reference's PirError(String), src/pir/error.rs:6). Refusals happen on the
+ * host before any device work, with the reference's message text where the same
+ * condition exists there.
+ *
+ * Polynomials are arrays of ring_dim u64 coefficients in [0, q), coefficient form,
+ * limb-major as the reference's Poly::coeffs() (src/math/poly.rs:51-58).
+ *
+ * Object lifetimes: a context owns its shards' device table; shards and seats must
+ * be destroyed before their context. Calls on one context are serialised by an
+ * internal lock, except inspire_respond_batch_device, which only reads the context
+ * and may run concurrently on distinct workspaces and streams.
+ */
+#ifndef INSPIRE_GPU_H
+#define INSPIRE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define INSPIRE_OK 0
+#define INSPIRE_ERR_INVALID 1 /* refused input: shape, width, modulus, unknown shard */
+#define INSPIRE_ERR_DEVICE 2  /* HIP runtime failure */
+#define INSPIRE_ERR_STORE 3   /* a shard's columns no longer match its packing store */
+
+#define INSPIRE_Q 1152921504606830593ULL      /* 2^60 - 2^14 + 1, src/math/mod_q.rs:7 */
+#define INSPIRE_Q36 68718428161ULL            /* MOD_SWITCH_TARGET_36BIT, src/pir/mod_switch.rs:47 */
+
+typedef struct inspire_ctx inspire_ctx;
+typedef struct inspire_shard inspire_shard;
+typedef struct inspire_seat inspire_seat;
+typedef struct inspire_workspace inspire_workspace;
+
+/* The fields of InspireParams the respond path reads (src/params.rs:120-190). */
+typedef struct {
+    uint32_t ring_dim;                    /* 256 or 2048 */
+    uint64_t q;                           /* must be INSPIRE_Q */
+    uint64_t gadget_base;                 /* must be 2^20 */
+    uint32_t packing_gadget_len;          /* must be 3 */
+    uint32_t approximate_packing_gadget;  /* 1: the key has no row for the lowest digit */
+} inspire_params;
+
+/* PackParams::try_new + OfflinePackingKeys::generate for one CRS and shard width,
+ * i.e. ServerInspiringCache::new (src/pir/respond.rs:926-941,
+ * src/inspiring/inspiring2.rs:171-286, 537-598). num_columns is gamma. */
+int inspire_ctx_create(const inspire_params *params, const uint8_t w_seed[32], uint32_t num_columns,
+                       inspire_ctx **out, char *err, size_t err_cap);
+void inspire_ctx_destroy(inspire_ctx *ctx);
+/* key rows the seats must carry: packing_gadget_len minus the dropped digits
+ * (PackParams::key_rows, src/inspiring/inspiring2.rs:289-291) */
+uint32_t inspire_ctx_key_rows(const inspire_ctx *ctx);
+
+/* Upload one shard of the EncodedDatabase (ShardData, src/pir/setup.rs:280-289).
+ * columns: num_columns x ring_dim column polynomials (ShardData::polynomials), each
+ * coefficient a 16-bit column value (src/pir/encode_db.rs:76-159). The shard id is
+ * its dense position, as respond's require_dense_shard demands
+ * (src/pir/respond.rs:442-462). */
+int inspire_shard_upload_columns(inspire_ctx *ctx, uint32_t shard_id, const uint64_t *columns,
+                                 uint32_t num_columns, inspire_shard **out, char *err, size_t err_cap);
+/* The same from raw records: record m occupies records[m*entry_size ..], column j of
+ * record m is entry[2j] | entry[2j+1] << 8 (extract_column_value,
+ * src/pir/encode_db.rs:146-159). num_records <= ring_dim; missing rows are zero. */
+int inspire_shard_upload_records(inspire_ctx *ctx, uint32_t shard_id, const uint8_t *records,
+                                 size_t num_records, size_t entry_size, inspire_shard **out, char *err,
+                                 size_t err_cap);
+/* ShardData::precompute_packing (src/pir/setup.rs:335-340): build the step-1 store
+ * on the device, with keyed column tags under a fresh OS-entropy key. */
+int inspire_shard_precompute_packing(inspire_ctx *ctx, inspire_shard *shard, char *err, size_t err_cap);
+int inspire_shard_has_store(const inspire_shard *shard);
+/* Overwrite one column value in place without rebuilding the store (a shard edited
+ * after its store was built; tests/shard_packing_store.rs:328-368). The next respond
+ * against this shard is refused with INSPIRE_ERR_STORE. */
+int inspire_shard_write_value(inspire_shard *shard, uint32_t record, uint32_t column, uint16_t value, char *err,
+                              size_t err_cap);
+/* HBM bytes held by the shard: records + store + tag key + tags. */
+size_t inspire_shard_resident_bytes(const inspire_shard *shard);
+void inspire_shard_destroy(inspire_shard *shard);
+
+/* PackingKeySeat::from_wire (src/pir/session.rs:592-624): y_body is rows x ring_dim
+ * coefficient-form key rows; num_to_pack is the key's gamma. */
+int inspire_seat_create(inspire_ctx *ctx, const uint64_t *y_body, uint32_t rows, uint32_t num_to_pack,
+                        inspire_seat **out, char *err, size_t err_cap);
+size_t inspire_seat_resident_bytes(const inspire_seat *seat);
+void inspire_seat_destroy(inspire_seat *seat);
+
+/* respond_seeded_batch (src/pir/respond.rs:1189-1202) under one seat: query k is
+ * shard_ids[k], the 32-byte seed of its RGSW row (seeds + 32k, SeededRlweCiphertext,
+ * src/rlwe/types.rs:149-166) and that row's b (b + k*ring_dim). Outputs per query:
+ * a_out + k*ring_dim receives a_hat, b_prefix_out + k*gamma the first gamma
+ * coefficients of b (the packed response's prefix, src/pir/respond.rs:132-160).
+ * switch_modulus = 0 returns residues mod q; otherwise each coefficient is switched
+ * as mod_switch_poly (src/pir/mod_switch.rs:148-175) does. A refused query fails the
+ * whole batch with no partial output (src/pir/respond.rs:1185-1188). Host buffers. */
+int inspire_respond_batch(inspire_ctx *ctx, const inspire_seat *seat, const uint32_t *shard_ids,
+                          const uint8_t *seeds, const uint64_t *b, size_t count, uint64_t switch_modulus,
+                          uint64_t *a_out, uint64_t *b_prefix_out, char *err, size_t err_cap);
+
+/* inspire_respond_batch for unseeded queries (ClientQuery, src/pir/query.rs:103-117):
+ * a + k*ring_dim is query k's expanded RGSW row a instead of a seed, as
+ * respond_inspiring_cached answers it (src/pir/respond.rs:1022-1041). */
+int inspire_respond_batch_expanded(inspire_ctx *ctx, const inspire_seat *seat, const uint32_t *shard_ids,
+                                   const uint64_t *a, const uint64_t *b, size_t count, uint64_t switch_modulus,
+                                   uint64_t *a_out, uint64_t *b_prefix_out, char *err, size_t err_cap);
+
+/* Device-resident batches for serving loops. A workspace holds the scratch of one
+ * in-flight batch of up to max_count queries. */
+int inspire_workspace_create(inspire_ctx *ctx, size_t max_count, inspire_workspace **out, char *err,
+                             size_t err_cap);
+void inspire_workspace_destroy(inspire_workspace *ws);
+/* inspire_respond_batch with every buffer in device memory, enqueued on `stream`
+ * (a hipStream_t, NULL = default stream) and not waited for. Every shard named must
+ * already hold a store. Per-query status lands in the workspace: read it with
+ * inspire_workspace_status after the stream completes. */
+int inspire_respond_batch_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws,
+                                 const uint32_t *d_shard_ids, const uint8_t *d_seeds, const uint64_t *d_b,
+                                 size_t count, uint64_t switch_modulus, uint64_t *d_a_out,
+                                 uint64_t *d_b_prefix_out, void *stream, char *err, size_t err_cap);
+/* inspire_respond_batch_device, then waits: kernel_ms[0] receives the DB-scan kernel's
+ * duration and kernel_ms[1] the recursion kernel's, from HIP events recorded on
+ * `stream` around each launch. For measurement. */
+int inspire_respond_batch_device_timed(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws,
+                                       const uint32_t *d_shard_ids, const uint8_t *d_seeds, const uint64_t *d_b,
+                                       size_t count, uint64_t switch_modulus, uint64_t *d_a_out,
+                                       uint64_t *d_b_prefix_out, void *stream, float kernel_ms[2], char *err,
+                                       size_t err_cap);
+/* Copies count statuses (0 = answered, c+1 = column c failed its store tag). */
+int inspire_workspace_status(inspire_workspace *ws, size_t count, int32_t *status, char *err, size_t err_cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,706 @@
+// Host runtime behind include/inspire_gpu.h: parameter checks, HBM-resident objects,
+// and the launches of the respond pipeline.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/inspire_gpu.h"
+#include "device.hpp"
+#include "kernels.hpp"
+#include "modq.hpp"
+
+using namespace inspire;
+typedef unsigned __int128 u128;
+
+namespace {
+
+void set_err(char *err, size_t cap, const char *fmt, ...) {
+    if (!err || cap == 0) return;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(err, cap, fmt, ap);
+    va_end(ap);
+}
+
+uint64_t powmod(uint64_t b, uint64_t e, uint64_t m) {
+    uint64_t r = 1;
+    b %= m;
+    while (e) {
+        if (e & 1) r = (uint64_t)((u128)r * b % m);
+        b = (uint64_t)((u128)b * b % m);
+        e >>= 1;
+    }
+    return r;
+}
+uint64_t inv_mod(uint64_t a) { return powmod(a, kQ - 2, kQ); }
+
+// psi: a primitive 2n-th root of unity, g^((q-1)/2n) for the least quadratic
+// non-residue g (NttContext::find_primitive_root, src/math/ntt.rs:743-752).
+uint64_t find_psi(uint64_t two_n) {
+    uint64_t ex = (kQ - 1) / two_n;
+    for (uint64_t g = 2; g < 65536; g++) {
+        uint64_t c = powmod(g, ex, kQ);
+        if (powmod(c, two_n, kQ) == 1 && powmod(c, two_n / 2, kQ) != 1) return c;
+    }
+    return 0;
+}
+
+// ladder[m] = psi^brv(m) built as the reference does (ntt.rs:797-834), standard form.
+std::vector<uint64_t> ladder(size_t n, uint64_t psi) {
+    std::vector<uint64_t> f(n, 0);
+    if (n > 1) f[1] = 1;
+    for (size_t m = 1; m < n; m++) {
+        if ((m & (m - 1)) == 0)
+            f[m] = powmod(psi, n / (2 * m), kQ);
+        else
+            f[m] = (uint64_t)((u128)f[m & (m - 1)] * f[m & (~m + 1)] % kQ);
+    }
+    return f;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    hipError_t alloc(size_t b) {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        bytes = b;
+        return hipMalloc(&p, b ? b : 8);
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+bool os_random(void *out, size_t len) {
+    std::ifstream f("/dev/urandom", std::ios::binary);
+    if (!f) return false;
+    f.read(static_cast<char *>(out), (std::streamsize)len);
+    return (size_t)f.gcount() == len;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            set_err(err, err_cap, "HIP failure in %s: %s", #expr, hipGetErrorString(e_)); \
+            return INSPIRE_ERR_DEVICE;                                                 \
+        }                                                                              \
+    } while (0)
+
+}  // namespace
+
+struct inspire_shard {
+    inspire_ctx *ctx = nullptr;
+    uint32_t id = 0;
+    DevBuf records, store, tag_key, tags;
+    bool has_store = false;
+};
+
+struct inspire_seat {
+    inspire_ctx *ctx = nullptr;
+    uint32_t rows = 0, num_to_pack = 0;
+    DevBuf yprime;
+};
+
+struct inspire_workspace {
+    inspire_ctx *ctx = nullptr;
+    size_t max_count = 0;
+    DevBuf scan_part, status;
+    // staging for the host-buffer entry point
+    DevBuf ids, seeds, a_in, b, a_out, b_out;
+};
+
+struct inspire_ctx {
+    std::mutex mu;
+    int logn = 0, n = 0, gamma = 0, rows = 0, dropped = 0;
+    uint8_t w_seed[32];
+    DevBuf psi, psi_inv, wprime, gpow, gpow_back, shard_table, col_scratch;
+    uint64_t inv_n = 0, inv_n_gamma = 0, kx = 0, k0 = 0, k1 = 0;
+    std::vector<ShardDev> shards;        // dense by id
+    std::vector<inspire_shard *> owners;
+    bool table_dirty = true;
+    inspire_workspace *default_ws = nullptr;
+
+    int ks() const { return n >= 2048 ? 8 : 1; }
+
+    CtxDev dev() const {
+        CtxDev c;
+        c.logn = logn; c.n = n; c.gamma = gamma; c.rows = rows; c.exact = dropped == 0;
+        c.psi = psi.as<uint64_t>(); c.psi_inv = psi_inv.as<uint64_t>();
+        c.wprime = wprime.as<uint64_t>();
+        c.gpow = gpow.as<uint32_t>(); c.gpow_back = gpow_back.as<uint32_t>();
+        c.inv_n_gamma = inv_n_gamma; c.kx = kx; c.k0 = k0; c.k1 = k1;
+        c.shards = shard_table.as<ShardDev>();
+        c.num_shards = (int)shards.size();
+        return c;
+    }
+};
+
+namespace {
+
+int sync_table(inspire_ctx *ctx, char *err, size_t err_cap) {
+    if (!ctx->table_dirty) return INSPIRE_OK;
+    size_t bytes = ctx->shards.size() * sizeof(ShardDev);
+    if (ctx->shard_table.bytes < bytes || !ctx->shard_table.p) HIP_TRY(ctx->shard_table.alloc(bytes * 2 + 64));
+    if (bytes) HIP_TRY(hipMemcpy(ctx->shard_table.p, ctx->shards.data(), bytes, hipMemcpyHostToDevice));
+    ctx->table_dirty = false;
+    return INSPIRE_OK;
+}
+
+int ws_reserve(inspire_workspace *ws, size_t count, char *err, size_t err_cap) {
+    inspire_ctx *ctx = ws->ctx;
+    if (count <= ws->max_count && ws->scan_part.p) return INSPIRE_OK;
+    size_t c = count < 1 ? 1 : count;
+    HIP_TRY(ws->scan_part.alloc(c * ctx->ks() * ctx->gamma * 4 * sizeof(uint64_t)));
+    HIP_TRY(ws->status.alloc(c * sizeof(int32_t)));
+    ws->max_count = c;
+    return INSPIRE_OK;
+}
+
+int build_store(inspire_ctx *ctx, inspire_shard *sh, char *err, size_t err_cap) {
+    const size_t n = ctx->n, g = ctx->gamma;
+    std::vector<uint64_t> key(n);
+    for (size_t k = 0; k < n; k++) {
+        // a key word below 2^61 - 1, drawn as the reference draws its tag key
+        // (shard_store.rs:127-135): 61 random bits, rejected at or above the prime
+        for (;;) {
+            uint64_t v;
+            if (!os_random(&v, 8)) {
+                set_err(err, err_cap, "OS entropy unavailable for ShardPackingStore::build");
+                return INSPIRE_ERR_DEVICE;
+            }
+            v >>= 3;
+            if (v < kTagPrime) { key[k] = v; break; }
+        }
+    }
+    if (!sh->store.p) HIP_TRY(sh->store.alloc(g * n * 8));
+    if (!sh->tag_key.p) HIP_TRY(sh->tag_key.alloc(n * 8));
+    if (!sh->tags.p) HIP_TRY(sh->tags.alloc(g * 8));
+    if (!ctx->col_scratch.p) HIP_TRY(ctx->col_scratch.alloc(g * n * 2));
+    HIP_TRY(hipMemcpy(sh->tag_key.p, key.data(), n * 8, hipMemcpyHostToDevice));
+    int rc = sync_table(ctx, err, err_cap);
+    if (rc) return rc;
+    HIP_TRY(launch_store_build(ctx->dev(), sh->records.as<uint16_t>(), ctx->col_scratch.as<uint16_t>(),
+                               sh->tag_key.as<uint64_t>(), sh->store.as<uint64_t>(), sh->tags.as<uint64_t>(), 0));
+    HIP_TRY(hipDeviceSynchronize());
+    sh->has_store = true;
+    ShardDev &d = ctx->shards[sh->id];
+    d.store = sh->store.as<uint64_t>();
+    d.tag_key = sh->tag_key.as<uint64_t>();
+    d.tags = sh->tags.as<uint64_t>();
+    ctx->table_dirty = true;
+    return sync_table(ctx, err, err_cap);
+}
+
+int register_shard(inspire_ctx *ctx, uint32_t id, const std::vector<uint16_t> &rec, inspire_shard **out, char *err,
+                   size_t err_cap) {
+    if (id < ctx->owners.size() && ctx->owners[id]) {
+        set_err(err, err_cap, "shard %u is already resident; destroy it before uploading another at that position", id);
+        return INSPIRE_ERR_INVALID;
+    }
+    inspire_shard *sh = new (std::nothrow) inspire_shard();
+    if (!sh) { set_err(err, err_cap, "out of host memory"); return INSPIRE_ERR_DEVICE; }
+    sh->ctx = ctx;
+    sh->id = id;
+    hipError_t e = sh->records.alloc(rec.size() * 2);
+    if (e == hipSuccess) e = hipMemcpy(sh->records.p, rec.data(), rec.size() * 2, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_err(err, err_cap, "HIP failure uploading shard %u: %s", id, hipGetErrorString(e));
+        delete sh;
+        return INSPIRE_ERR_DEVICE;
+    }
+    if (ctx->shards.size() <= id) {
+        ctx->shards.resize(id + 1, ShardDev{nullptr, nullptr, nullptr, nullptr});
+        ctx->owners.resize(id + 1, nullptr);
+    }
+    ctx->shards[id] = ShardDev{sh->records.as<uint16_t>(), nullptr, nullptr, nullptr};
+    ctx->owners[id] = sh;
+    ctx->table_dirty = true;
+    *out = sh;
+    return INSPIRE_OK;
+}
+
+// Host-side refusals, before any launch (the reference checks every query's shape
+// and shard before packing: respond.rs:409-611).
+int check_batch(inspire_ctx *ctx, const inspire_seat *seat, const uint32_t *ids, size_t count, char *err,
+                size_t err_cap) {
+    if (!seat || seat->ctx != ctx) {
+        set_err(err, err_cap, "packing-key seat was built for another context");
+        return INSPIRE_ERR_INVALID;
+    }
+    if (seat->num_to_pack != (uint32_t)ctx->gamma) {
+        set_err(err, err_cap,
+                "packing key was built for gamma = %u but the server packs gamma = %d; the client must re-derive "
+                "its packing keys against the current CRS",
+                seat->num_to_pack, ctx->gamma);
+        return INSPIRE_ERR_INVALID;
+    }
+    for (size_t k = 0; ids && k < count; k++) {
+        uint32_t id = ids[k];
+        if (id >= ctx->owners.size() || !ctx->owners[id]) {
+            set_err(err, err_cap,
+                    "respond_seeded_with_seat: shard %u not found at dense position %u; encoded database has %zu "
+                    "shards",
+                    id, id, ctx->owners.size());
+            return INSPIRE_ERR_INVALID;
+        }
+    }
+    return INSPIRE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int inspire_ctx_create(const inspire_params *params, const uint8_t w_seed[32], uint32_t num_columns,
+                       inspire_ctx **out, char *err, size_t err_cap) {
+    if (!params || !w_seed || !out) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    *out = nullptr;
+    const uint32_t n = params->ring_dim;
+    if (n != 256 && n != 2048) {
+        set_err(err, err_cap, "ring_dim %u has no gfx950 kernel here; built ring dimensions are 256 and 2048", n);
+        return INSPIRE_ERR_INVALID;
+    }
+    if (params->q != kQ) {
+        set_err(err, err_cap, "modulus %llu is not the single 60-bit prime 2^60 - 2^14 + 1 these kernels reduce by",
+                (unsigned long long)params->q);
+        return INSPIRE_ERR_INVALID;
+    }
+    if (params->gadget_base != (1ULL << 20) || params->packing_gadget_len != 3) {
+        set_err(err, err_cap, "packing gadget (base %llu, len %u) unsupported: the kernels decompose base 2^20, 3 digits",
+                (unsigned long long)params->gadget_base, params->packing_gadget_len);
+        return INSPIRE_ERR_INVALID;
+    }
+    bool legal = num_columns != 0 && (num_columns & (num_columns - 1)) == 0 && num_columns <= n / 2 &&
+                 num_columns <= 1024;
+    if (!legal) {
+        set_err(err, err_cap,
+                "shard column count is not a legal InspiRING width: InspiRING packing width %u is illegal at ring_dim "
+                "%u: the width must be a power of two in [1, %u] so that the automorphism indices form a subgroup of "
+                "Z*_%u of order %u",
+                num_columns, n, n / 2, 2 * n, num_columns);
+        return INSPIRE_ERR_INVALID;
+    }
+    inspire_ctx *ctx = new (std::nothrow) inspire_ctx();
+    if (!ctx) { set_err(err, err_cap, "out of host memory"); return INSPIRE_ERR_DEVICE; }
+    ctx->n = (int)n;
+    ctx->logn = n == 256 ? 8 : 11;
+    ctx->gamma = (int)num_columns;
+    ctx->dropped = params->approximate_packing_gadget ? 1 : 0;
+    ctx->rows = 3 - ctx->dropped;
+    memcpy(ctx->w_seed, w_seed, 32);
+
+    const uint64_t two_n = 2ULL * n;
+    uint64_t psi = find_psi(two_n);
+    std::vector<uint64_t> fwd = ladder(n, psi), inv = ladder(n, inv_mod(psi));
+    uint64_t g = two_n / num_columns + 1;  // PackParams::partial_generator (inspiring2.rs:196-198)
+    std::vector<uint64_t> pows(n);
+    uint64_t v = 1;
+    for (uint32_t i = 0; i < n; i++) { pows[i] = v; v = v * g % two_n; }
+    std::vector<uint32_t> gpow(num_columns), gback(num_columns);
+    for (uint32_t i = 0; i < num_columns; i++) {
+        gpow[i] = (uint32_t)pows[i];
+        gback[i] = (uint32_t)pows[(n - i) % n];
+    }
+    const uint64_t base = 1ULL << 20;
+    uint64_t inv_b = inv_mod(base), inv_b2 = (uint64_t)((u128)inv_b * inv_b % kQ);
+    ctx->inv_n = inv_mod(n);
+    ctx->inv_n_gamma = inv_mod((uint64_t)((u128)n * num_columns % kQ));
+    ctx->kx = (uint64_t)((u128)n * inv_b2 % kQ);
+    ctx->k0 = kQ - inv_b2;
+    ctx->k1 = kQ - inv_b;
+
+    hipError_t e = hipSuccess;
+    DevBuf d_seed;
+    if (e == hipSuccess) e = ctx->psi.alloc(n * 8);
+    if (e == hipSuccess) e = ctx->psi_inv.alloc(n * 8);
+    if (e == hipSuccess) e = ctx->gpow.alloc(num_columns * 4);
+    if (e == hipSuccess) e = ctx->gpow_back.alloc(num_columns * 4);
+    if (e == hipSuccess) e = ctx->wprime.alloc((size_t)ctx->rows * n * 8);
+    if (e == hipSuccess) e = d_seed.alloc(32);
+    if (e == hipSuccess) e = hipMemcpy(ctx->psi.p, fwd.data(), n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ctx->psi_inv.p, inv.data(), n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ctx->gpow.p, gpow.data(), num_columns * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ctx->gpow_back.p, gback.data(), num_columns * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_seed.p, w_seed, 32, hipMemcpyHostToDevice);
+    // kept_mask_rows (inspiring2.rs:839-855): stream 0 for the exact gadget, gamma otherwise
+    if (e == hipSuccess)
+        e = launch_expand_mask(ctx->logn, ctx->rows, d_seed.as<uint8_t>(), ctx->dropped ? num_columns : 0,
+                               ctx->dropped, ctx->psi.as<uint64_t>(), ctx->inv_n, ctx->wprime.as<uint64_t>(), 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        set_err(err, err_cap, "HIP failure creating the context: %s", hipGetErrorString(e));
+        delete ctx;
+        return INSPIRE_ERR_DEVICE;
+    }
+    *out = ctx;
+    return INSPIRE_OK;
+}
+
+void inspire_ctx_destroy(inspire_ctx *ctx) {
+    if (!ctx) return;
+    if (ctx->default_ws) inspire_workspace_destroy(ctx->default_ws);
+    delete ctx;
+}
+
+uint32_t inspire_ctx_key_rows(const inspire_ctx *ctx) { return ctx ? (uint32_t)ctx->rows : 0; }
+
+int inspire_shard_upload_columns(inspire_ctx *ctx, uint32_t shard_id, const uint64_t *columns, uint32_t num_columns,
+                                 inspire_shard **out, char *err, size_t err_cap) {
+    if (!ctx || !columns || !out) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const size_t n = ctx->n, g = ctx->gamma;
+    if (num_columns != g) {
+        set_err(err, err_cap,
+                "respond_seeded_inspiring: shard %u column-count mismatch: got %u, expected %zu from the CRS; refusing "
+                "malformed encoded state before it returns wrong plaintext",
+                shard_id, num_columns, g);
+        return INSPIRE_ERR_INVALID;
+    }
+    std::vector<uint16_t> rec(n * g);
+    for (size_t j = 0; j < g; j++)
+        for (size_t m = 0; m < n; m++) {
+            uint64_t v = columns[j * n + m];
+            if (v > 0xffff) {
+                set_err(err, err_cap,
+                        "shard %u column %zu coefficient %zu is %llu, not a 16-bit column value of an encoded record",
+                        shard_id, j, m, (unsigned long long)v);
+                return INSPIRE_ERR_INVALID;
+            }
+            rec[m * g + j] = (uint16_t)v;
+        }
+    return register_shard(ctx, shard_id, rec, out, err, err_cap);
+}
+
+int inspire_shard_upload_records(inspire_ctx *ctx, uint32_t shard_id, const uint8_t *records, size_t num_records,
+                                 size_t entry_size, inspire_shard **out, char *err, size_t err_cap) {
+    if (!ctx || (!records && num_records) || !out) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const size_t n = ctx->n, g = ctx->gamma;
+    if ((entry_size + 1) / 2 != g) {
+        set_err(err, err_cap,
+                "respond_seeded_inspiring: shard %u column-count mismatch: got %zu, expected %zu from the CRS; refusing "
+                "malformed encoded state before it returns wrong plaintext",
+                shard_id, (entry_size + 1) / 2, g);
+        return INSPIRE_ERR_INVALID;
+    }
+    if (num_records > n) {
+        set_err(err, err_cap, "shard %u holds %zu records, more than ring_dim %zu rows", shard_id, num_records, n);
+        return INSPIRE_ERR_INVALID;
+    }
+    std::vector<uint16_t> rec(n * g, 0);
+    for (size_t m = 0; m < num_records; m++) {
+        const uint8_t *e = records + m * entry_size;
+        for (size_t j = 0; j < g; j++) {
+            size_t o = 2 * j;
+            uint16_t v = 0;
+            if (o + 1 < entry_size) v = (uint16_t)(e[o] | (e[o + 1] << 8));
+            else if (o < entry_size) v = e[o];
+            rec[m * g + j] = v;
+        }
+    }
+    return register_shard(ctx, shard_id, rec, out, err, err_cap);
+}
+
+int inspire_shard_precompute_packing(inspire_ctx *ctx, inspire_shard *shard, char *err, size_t err_cap) {
+    if (!ctx || !shard || shard->ctx != ctx) { set_err(err, err_cap, "shard belongs to another context"); return INSPIRE_ERR_INVALID; }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return build_store(ctx, shard, err, err_cap);
+}
+
+int inspire_shard_has_store(const inspire_shard *shard) { return shard && shard->has_store ? 1 : 0; }
+
+int inspire_shard_write_value(inspire_shard *shard, uint32_t record, uint32_t column, uint16_t value, char *err,
+                              size_t err_cap) {
+    if (!shard) { set_err(err, err_cap, "null shard"); return INSPIRE_ERR_INVALID; }
+    inspire_ctx *ctx = shard->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (record >= (uint32_t)ctx->n || column >= (uint32_t)ctx->gamma) {
+        set_err(err, err_cap, "record %u column %u outside the %d x %d shard", record, column, ctx->n, ctx->gamma);
+        return INSPIRE_ERR_INVALID;
+    }
+    size_t off = ((size_t)record * ctx->gamma + column) * 2;
+    HIP_TRY(hipMemcpy(static_cast<char *>(shard->records.p) + off, &value, 2, hipMemcpyHostToDevice));
+    return INSPIRE_OK;
+}
+
+size_t inspire_shard_resident_bytes(const inspire_shard *shard) {
+    if (!shard) return 0;
+    return shard->records.bytes + (shard->has_store ? shard->store.bytes + shard->tag_key.bytes + shard->tags.bytes : 0);
+}
+
+void inspire_shard_destroy(inspire_shard *shard) {
+    if (!shard) return;
+    inspire_ctx *ctx = shard->ctx;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (shard->id < ctx->owners.size() && ctx->owners[shard->id] == shard) {
+            ctx->owners[shard->id] = nullptr;
+            ctx->shards[shard->id] = ShardDev{nullptr, nullptr, nullptr, nullptr};
+            ctx->table_dirty = true;
+        }
+    }
+    (void)hipDeviceSynchronize();
+    delete shard;
+}
+
+int inspire_seat_create(inspire_ctx *ctx, const uint64_t *y_body, uint32_t rows, uint32_t num_to_pack,
+                        inspire_seat **out, char *err, size_t err_cap) {
+    if (!ctx || !y_body || !out) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const size_t n = ctx->n;
+    if (num_to_pack != (uint32_t)ctx->gamma) {
+        set_err(err, err_cap,
+                "packing key was built for gamma = %u but the server packs gamma = %d; the client must re-derive its "
+                "packing keys against the current CRS",
+                num_to_pack, ctx->gamma);
+        return INSPIRE_ERR_INVALID;
+    }
+    if (rows != (uint32_t)ctx->rows) {
+        set_err(err, err_cap,
+                "packing-key seat is for another ring or gadget: it holds %u key rows on ring_dim %zu, the server packs "
+                "%d rows on ring_dim %zu. The client must re-derive its packing keys against the current CRS.",
+                rows, n, ctx->rows, n);
+        return INSPIRE_ERR_INVALID;
+    }
+    for (size_t i = 0; i < rows * n; i++)
+        if (y_body[i] >= kQ) {
+            set_err(err, err_cap, "Poly decode refused: limb 0 coefficient %zu is %llu, not canonical for modulus %llu",
+                    i % n, (unsigned long long)y_body[i], (unsigned long long)kQ);
+            return INSPIRE_ERR_INVALID;
+        }
+    inspire_seat *seat = new (std::nothrow) inspire_seat();
+    if (!seat) { set_err(err, err_cap, "out of host memory"); return INSPIRE_ERR_DEVICE; }
+    seat->ctx = ctx;
+    seat->rows = rows;
+    seat->num_to_pack = num_to_pack;
+    DevBuf tmp;
+    hipError_t e = seat->yprime.alloc(rows * n * 8);
+    if (e == hipSuccess) e = tmp.alloc(rows * n * 8);
+    if (e == hipSuccess) e = hipMemcpy(tmp.p, y_body, rows * n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = launch_seat(ctx->logn, (int)rows, tmp.as<uint64_t>(), ctx->psi.as<uint64_t>(), ctx->inv_n,
+                        seat->yprime.as<uint64_t>(), 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        set_err(err, err_cap, "HIP failure seating the packing key: %s", hipGetErrorString(e));
+        delete seat;
+        return INSPIRE_ERR_DEVICE;
+    }
+    *out = seat;
+    return INSPIRE_OK;
+}
+
+size_t inspire_seat_resident_bytes(const inspire_seat *seat) { return seat ? seat->yprime.bytes : 0; }
+
+void inspire_seat_destroy(inspire_seat *seat) {
+    if (!seat) return;
+    (void)hipDeviceSynchronize();
+    delete seat;
+}
+
+int inspire_workspace_create(inspire_ctx *ctx, size_t max_count, inspire_workspace **out, char *err, size_t err_cap) {
+    if (!ctx || !out) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    inspire_workspace *ws = new (std::nothrow) inspire_workspace();
+    if (!ws) { set_err(err, err_cap, "out of host memory"); return INSPIRE_ERR_DEVICE; }
+    ws->ctx = ctx;
+    int rc = ws_reserve(ws, max_count, err, err_cap);
+    if (rc) { delete ws; return rc; }
+    *out = ws;
+    return INSPIRE_OK;
+}
+
+void inspire_workspace_destroy(inspire_workspace *ws) {
+    if (!ws) return;
+    (void)hipDeviceSynchronize();
+    delete ws;
+}
+
+static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws,
+                          const uint32_t *d_shard_ids, const uint8_t *d_seeds, const uint64_t *d_b, size_t count,
+                          uint64_t switch_modulus, uint64_t *d_a_out, uint64_t *d_b_prefix_out, void *stream,
+                          float *timing, char *err, size_t err_cap) {
+    if (!ctx || !ws || ws->ctx != ctx) { set_err(err, err_cap, "workspace belongs to another context"); return INSPIRE_ERR_INVALID; }
+    int rc = check_batch(ctx, seat, nullptr, count, err, err_cap);
+    if (rc) return rc;
+    if (count > ws->max_count) {
+        set_err(err, err_cap, "batch of %zu queries exceeds the workspace's %zu", count, ws->max_count);
+        return INSPIRE_ERR_INVALID;
+    }
+    if (switch_modulus >= (1ULL << 37)) {
+        set_err(err, err_cap, "switch modulus %llu is above the 37-bit targets this path implements",
+                (unsigned long long)switch_modulus);
+        return INSPIRE_ERR_INVALID;
+    }
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        rc = sync_table(ctx, err, err_cap);
+        if (rc) return rc;
+    }
+    BatchDev b;
+    b.count = (int)count;
+    b.shard_ids = d_shard_ids;
+    b.seeds = d_seeds;
+    b.a_in = nullptr;
+    b.b = d_b;
+    b.yprime = seat->yprime.as<uint64_t>();
+    b.scan_part = ws->scan_part.as<uint64_t>();
+    b.ks = ctx->ks();
+    b.a_out = d_a_out;
+    b.b_out = d_b_prefix_out;
+    b.status = ws->status.as<int32_t>();
+    b.switch_modulus = switch_modulus;
+    if (!timing) {
+        HIP_TRY(launch_respond(ctx->dev(), b, static_cast<hipStream_t>(stream)));
+        return INSPIRE_OK;
+    }
+    // per-kernel durations from events on the launch stream
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipEvent_t ev[3];
+    for (int i = 0; i < 3; i++) HIP_TRY(hipEventCreate(&ev[i]));
+    hipError_t e = hipEventRecord(ev[0], st);
+    if (e == hipSuccess) e = launch_scan_only(ctx->dev(), b, st);
+    if (e == hipSuccess) e = hipEventRecord(ev[1], st);
+    if (e == hipSuccess) e = launch_chain_only(ctx->dev(), b, st);
+    if (e == hipSuccess) e = hipEventRecord(ev[2], st);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[2]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&timing[0], ev[0], ev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&timing[1], ev[1], ev[2]);
+    for (int i = 0; i < 3; i++) (void)hipEventDestroy(ev[i]);
+    HIP_TRY(e);
+    return INSPIRE_OK;
+}
+
+int inspire_respond_batch_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws,
+                                 const uint32_t *d_shard_ids, const uint8_t *d_seeds, const uint64_t *d_b,
+                                 size_t count, uint64_t switch_modulus, uint64_t *d_a_out, uint64_t *d_b_prefix_out,
+                                 void *stream, char *err, size_t err_cap) {
+    return respond_device(ctx, seat, ws, d_shard_ids, d_seeds, d_b, count, switch_modulus, d_a_out, d_b_prefix_out,
+                          stream, nullptr, err, err_cap);
+}
+
+int inspire_respond_batch_device_timed(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws,
+                                       const uint32_t *d_shard_ids, const uint8_t *d_seeds, const uint64_t *d_b,
+                                       size_t count, uint64_t switch_modulus, uint64_t *d_a_out,
+                                       uint64_t *d_b_prefix_out, void *stream, float kernel_ms[2], char *err,
+                                       size_t err_cap) {
+    if (!kernel_ms) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    return respond_device(ctx, seat, ws, d_shard_ids, d_seeds, d_b, count, switch_modulus, d_a_out, d_b_prefix_out,
+                          stream, kernel_ms, err, err_cap);
+}
+
+int inspire_workspace_status(inspire_workspace *ws, size_t count, int32_t *status, char *err, size_t err_cap) {
+    if (!ws || !status || count > ws->max_count) { set_err(err, err_cap, "bad status request"); return INSPIRE_ERR_INVALID; }
+    HIP_TRY(hipMemcpy(status, ws->status.p, count * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return INSPIRE_OK;
+}
+
+static int respond_host(inspire_ctx *ctx, const inspire_seat *seat, const uint32_t *shard_ids, const uint8_t *seeds,
+                        const uint64_t *a_in, const uint64_t *b, size_t count, uint64_t switch_modulus,
+                        uint64_t *a_out, uint64_t *b_prefix_out, char *err, size_t err_cap) {
+    if (!ctx || (count && (!shard_ids || (!seeds && !a_in) || !b || !a_out || !b_prefix_out))) {
+        set_err(err, err_cap, "null argument");
+        return INSPIRE_ERR_INVALID;
+    }
+    if (count == 0) return INSPIRE_OK;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    int rc = check_batch(ctx, seat, shard_ids, count, err, err_cap);
+    if (rc) return rc;
+    const size_t n = ctx->n, g = ctx->gamma;
+    for (size_t k = 0; k < count; k++)
+        for (size_t j = 0; j < n; j++)
+            if (b[k * n + j] >= kQ || (a_in && a_in[k * n + j] >= kQ)) {
+                set_err(err, err_cap,
+                        "Poly decode refused: limb 0 coefficient %zu is %llu, not canonical for modulus %llu", j,
+                        (unsigned long long)(b[k * n + j] >= kQ ? b[k * n + j] : a_in[k * n + j]),
+                        (unsigned long long)kQ);
+                return INSPIRE_ERR_INVALID;
+            }
+    if (switch_modulus >= (1ULL << 37)) {
+        set_err(err, err_cap, "switch modulus %llu is above the 37-bit targets this path implements",
+                (unsigned long long)switch_modulus);
+        return INSPIRE_ERR_INVALID;
+    }
+    // a shard decoded without its store packs through one built now: the bytes equal the
+    // per-query path's (shard_store.rs:1-16), so building lazily changes nothing observable
+    for (size_t k = 0; k < count; k++) {
+        inspire_shard *sh = ctx->owners[shard_ids[k]];
+        if (!sh->has_store) {
+            rc = build_store(ctx, sh, err, err_cap);
+            if (rc) return rc;
+        }
+    }
+    rc = sync_table(ctx, err, err_cap);
+    if (rc) return rc;
+    if (!ctx->default_ws) {
+        rc = inspire_workspace_create(ctx, count, &ctx->default_ws, err, err_cap);
+        if (rc) return rc;
+    }
+    inspire_workspace *ws = ctx->default_ws;
+    if (count > ws->max_count || !ws->ids.p) {
+        rc = ws_reserve(ws, count, err, err_cap);
+        if (rc) return rc;
+        HIP_TRY(ws->ids.alloc(ws->max_count * 4));
+        HIP_TRY(ws->seeds.alloc(ws->max_count * 32));
+        HIP_TRY(ws->a_in.alloc(ws->max_count * n * 8));
+        HIP_TRY(ws->b.alloc(ws->max_count * n * 8));
+        HIP_TRY(ws->a_out.alloc(ws->max_count * n * 8));
+        HIP_TRY(ws->b_out.alloc(ws->max_count * g * 8));
+    }
+    HIP_TRY(hipMemcpy(ws->ids.p, shard_ids, count * 4, hipMemcpyHostToDevice));
+    if (seeds) HIP_TRY(hipMemcpy(ws->seeds.p, seeds, count * 32, hipMemcpyHostToDevice));
+    if (a_in) HIP_TRY(hipMemcpy(ws->a_in.p, a_in, count * n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ws->b.p, b, count * n * 8, hipMemcpyHostToDevice));
+    BatchDev bd;
+    bd.count = (int)count;
+    bd.shard_ids = ws->ids.as<uint32_t>();
+    bd.seeds = ws->seeds.as<uint8_t>();
+    bd.a_in = a_in ? ws->a_in.as<uint64_t>() : nullptr;
+    bd.b = ws->b.as<uint64_t>();
+    bd.yprime = seat->yprime.as<uint64_t>();
+    bd.scan_part = ws->scan_part.as<uint64_t>();
+    bd.ks = ctx->ks();
+    bd.a_out = ws->a_out.as<uint64_t>();
+    bd.b_out = ws->b_out.as<uint64_t>();
+    bd.status = ws->status.as<int32_t>();
+    bd.switch_modulus = switch_modulus;
+    HIP_TRY(launch_respond(ctx->dev(), bd, 0));
+    HIP_TRY(hipDeviceSynchronize());
+    std::vector<int32_t> st(count);
+    HIP_TRY(hipMemcpy(st.data(), ws->status.p, count * 4, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < count; k++)
+        if (st[k] != 0) {
+            set_err(err, err_cap,
+                    "shard %u: packing store refused: column %d is not the column the store was built from. The "
+                    "shard changed after the build; rebuild the store before serving it",
+                    shard_ids[k], st[k] - 1);
+            return INSPIRE_ERR_STORE;
+        }
+    HIP_TRY(hipMemcpy(a_out, ws->a_out.p, count * n * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(b_prefix_out, ws->b_out.p, count * g * 8, hipMemcpyDeviceToHost));
+    return INSPIRE_OK;
+}
+
+int inspire_respond_batch(inspire_ctx *ctx, const inspire_seat *seat, const uint32_t *shard_ids, const uint8_t *seeds,
+                          const uint64_t *b, size_t count, uint64_t switch_modulus, uint64_t *a_out,
+                          uint64_t *b_prefix_out, char *err, size_t err_cap) {
+    if (count && !seeds) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    return respond_host(ctx, seat, shard_ids, seeds, nullptr, b, count, switch_modulus, a_out, b_prefix_out, err,
+                        err_cap);
+}
+
+int inspire_respond_batch_expanded(inspire_ctx *ctx, const inspire_seat *seat, const uint32_t *shard_ids,
+                                   const uint64_t *a, const uint64_t *b, size_t count, uint64_t switch_modulus,
+                                   uint64_t *a_out, uint64_t *b_prefix_out, char *err, size_t err_cap) {
+    if (count && !a) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    return respond_host(ctx, seat, shard_ids, nullptr, a, b, count, switch_modulus, a_out, b_prefix_out, err,
+                        err_cap);
+}
+
+}  // extern "C"

@@ -1,0 +1,28 @@
+// Host build of the device arithmetic headers, exported for the CPU test suite
+// (tests/test_device_arith_host.py checks them against big-integer arithmetic).
+#include <stdint.h>
+#include "chacha.hpp"
+#include "modq.hpp"
+
+using namespace inspire;
+
+extern "C" {
+uint64_t st_mulmod(uint64_t a, uint64_t b) { return mulmod(a, b); }
+uint64_t st_reduce120(uint64_t hi, uint64_t lo) { return reduce120(hi, lo); }
+void st_mul_wide(uint64_t a, uint64_t b, uint64_t *hi, uint64_t *lo) { *lo = mul_wide(a, b, *hi); }
+uint64_t st_acc_reduce(const uint64_t *a, const uint64_t *b, int count, uint64_t init) {
+    Acc128 acc;
+    acc.add(init);
+    for (int i = 0; i < count; i++) acc.mac(a[i], b[i]);
+    return acc.reduce();
+}
+uint64_t st_mod_switch(uint64_t c, uint64_t qn) { return mod_switch_coeff(c, qn); }
+uint64_t st_mod_p61(uint64_t hi, uint64_t lo) { return mod_p61(hi, lo); }
+uint32_t st_automorph_slot(uint32_t t, uint32_t j, int logn) { return automorph_slot(t, j, logn); }
+void st_chacha_block(const uint8_t *seed, uint64_t counter, uint64_t stream, uint32_t *out) {
+    uint32_t key[8];
+    seed_key(seed, key);
+    chacha20_block(key, counter, stream, out);
+}
+int st_uniform_q_draw(uint64_t v, uint64_t *out) { return uniform_q_draw(v, *out) ? 1 : 0; }
+}

@@ -1,0 +1,128 @@
+"""The device arithmetic headers (modq.hpp, chacha.hpp), compiled for the host, against
+big-integer arithmetic and the oracle. Runs on the CPU: it checks the exact code the
+kernels inline, not a restatement of it."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "inspire-rs_amd", "libinspire_selftest.so")
+Q = oracle.Q
+
+
+@pytest.fixture(scope="module")
+def st():
+    if not os.path.exists(LIB):
+        import subprocess
+        subprocess.run(["make", "-s", "-C", os.path.dirname(LIB), "libinspire_selftest.so"], check=True)
+    L = ctypes.CDLL(LIB)
+    u64, u32 = ctypes.c_uint64, ctypes.c_uint32
+    L.st_mulmod.argtypes = [u64, u64]
+    L.st_mulmod.restype = u64
+    L.st_reduce120.argtypes = [u64, u64]
+    L.st_reduce120.restype = u64
+    L.st_mul_wide.argtypes = [u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+    L.st_acc_reduce.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, u64]
+    L.st_acc_reduce.restype = u64
+    L.st_mod_switch.argtypes = [u64, u64]
+    L.st_mod_switch.restype = u64
+    L.st_mod_p61.argtypes = [u64, u64]
+    L.st_mod_p61.restype = u64
+    L.st_automorph_slot.argtypes = [u32, u32, ctypes.c_int]
+    L.st_automorph_slot.restype = u32
+    L.st_chacha_block.argtypes = [ctypes.c_char_p, u64, u64, ctypes.c_void_p]
+    L.st_uniform_q_draw.argtypes = [u64, ctypes.POINTER(u64)]
+    return L
+
+
+EDGES = [0, 1, 2, Q - 1, Q - 2, Q // 2, (1 << 32) - 1, 1 << 32, (1 << 59) + 12345, (1 << 60) - (1 << 14)]
+
+
+def test_mulmod_edges_and_random(st):
+    rng = np.random.default_rng(0)
+    vals = EDGES + [int(x) for x in rng.integers(0, Q, 300, dtype=np.uint64)]
+    for a in vals:
+        for b in vals[:40]:
+            assert st.st_mulmod(a, b) == a * b % Q
+
+
+def test_mul_wide_full_64bit(st):
+    rng = np.random.default_rng(1)
+    hi, lo = ctypes.c_uint64(), ctypes.c_uint64()
+    for a, b in [(2**64 - 1, 2**64 - 1), (2**64 - 1, Q)] + [tuple(int(x) for x in rng.integers(0, 2**63, 2, dtype=np.uint64)) for _ in range(200)]:
+        st.st_mul_wide(a, b, ctypes.byref(hi), ctypes.byref(lo))
+        assert (hi.value << 64) | lo.value == a * b
+
+
+def test_reduce120_range(st):
+    rng = np.random.default_rng(2)
+    for _ in range(2000):
+        x = int(rng.integers(0, 2**60, dtype=np.uint64)) * int(rng.integers(0, 2**60, dtype=np.uint64))
+        assert st.st_reduce120(x >> 64, x & (2**64 - 1)) == x % Q
+    top = (Q - 1) * (Q - 1)
+    assert st.st_reduce120(top >> 64, top & (2**64 - 1)) == top % Q
+
+
+def test_acc128_sums_of_products(st):
+    rng = np.random.default_rng(3)
+    for count in (1, 3, 4, 7, 16):
+        a = rng.integers(0, Q, count, dtype=np.uint64)
+        b = rng.integers(0, Q, count, dtype=np.uint64)
+        init = int(rng.integers(0, Q, dtype=np.uint64))
+        want = (init + sum(int(x) * int(y) for x, y in zip(a, b))) % Q
+        assert st.st_acc_reduce(a.ctypes.data, b.ctypes.data, count, init) == want
+    a = np.full(16, Q - 1, dtype=np.uint64)
+    assert st.st_acc_reduce(a.ctypes.data, a.ctypes.data, 16, Q - 1) == (Q - 1 + 16 * (Q - 1) ** 2) % Q
+
+
+def test_mod_switch_matches_oracle(st):
+    rng = np.random.default_rng(4)
+    vals = np.array(EDGES[:6] + [int(x) for x in rng.integers(0, Q, 2000, dtype=np.uint64)], dtype=np.uint64)
+    want = oracle.mod_switch(vals)
+    assert [st.st_mod_switch(int(v), oracle.Q36) for v in vals] == [int(w) for w in want]
+
+
+def test_mod_p61(st):
+    p = 2**61 - 1
+    rng = np.random.default_rng(5)
+    for _ in range(1000):
+        hi = int(rng.integers(0, 2**58, dtype=np.uint64))
+        lo = int(rng.integers(0, 2**63, dtype=np.uint64)) * 2 + 1
+        assert st.st_mod_p61(hi, lo) == ((hi << 64) | lo) % p
+
+
+@pytest.mark.parametrize("n,gamma", [(256, 16), (2048, 256)])
+def test_automorph_slot_matches_reference_tables(st, n, gamma):
+    p = oracle.Pack(n, gamma)
+    logn = n.bit_length() - 1
+    g = 2 * n // gamma + 1
+    for t in (1, g, pow(g, 7, 2 * n), 2 * n - 1):
+        tab = p.automorph_table(t)
+        assert [st.st_automorph_slot(t, j, logn) for j in range(n)] == list(tab)
+
+
+def test_chacha_and_uniform_draws_match_oracle(st):
+    seed = bytes(range(7, 39))
+    out = np.zeros(16, dtype=np.uint32)
+    want = oracle.expand_seed(seed, 64)
+    got = []
+    for blk in range(8):
+        st.st_chacha_block(seed, blk, 0, out.ctypes.data)
+        for u in range(8):
+            v = int(out[2 * u]) | (int(out[2 * u + 1]) << 32)
+            r = ctypes.c_uint64()
+            assert st.st_uniform_q_draw(v, ctypes.byref(r)) == 1
+            got.append(r.value)
+    assert got == [int(x) for x in want]
+    # the rejection zone edge: lo = v*q mod 2^64 must be <= (q << 4) - 1
+    r = ctypes.c_uint64()
+    zone = ((Q << 4) - 1) & (2**64 - 1)
+    qinv = pow(Q, -1, 2**64)
+    v_reject = ((zone + 1) * qinv) % 2**64  # v*q = zone+1 (mod 2^64): just outside
+    assert st.st_uniform_q_draw(v_reject, ctypes.byref(r)) == 0
+    v_accept = (zone * qinv) % 2**64
+    assert st.st_uniform_q_draw(v_accept, ctypes.byref(r)) == 1
