@@ -120,8 +120,9 @@ struct inspire_ctx {
     std::mutex mu;
     int logn = 0, n = 0, gamma = 0, rows = 0, dropped = 0;
     uint8_t w_seed[32];
-    DevBuf psi, psi_inv, wprime, gpow, gpow_back, shard_table, col_scratch;
-    uint64_t inv_n = 0, inv_n_gamma = 0, kx = 0, k0 = 0, k1 = 0;
+    DevBuf psi, psi_inv, twf, twi, wprime, gpow, gpow_back, shard_table, col_scratch;
+    uint64_t inv_n = 0, inv_n_gamma = 0;
+    Pair kx{0, 0}, k0{0, 0}, k1{0, 0};
     std::vector<ShardDev> shards;        // dense by id
     std::vector<inspire_shard *> owners;
     bool table_dirty = true;
@@ -133,7 +134,8 @@ struct inspire_ctx {
         CtxDev c;
         c.logn = logn; c.n = n; c.gamma = gamma; c.rows = rows; c.exact = dropped == 0;
         c.psi = psi.as<uint64_t>(); c.psi_inv = psi_inv.as<uint64_t>();
-        c.wprime = wprime.as<uint64_t>();
+        c.twf = twf.as<Pair>(); c.twi = twi.as<Pair>();
+        c.wprime = wprime.as<Pair>();
         c.gpow = gpow.as<uint32_t>(); c.gpow_back = gpow_back.as<uint32_t>();
         c.inv_n_gamma = inv_n_gamma; c.kx = kx; c.k0 = k0; c.k1 = k1;
         c.shards = shard_table.as<ShardDev>();
@@ -179,7 +181,7 @@ int build_store(inspire_ctx *ctx, inspire_shard *sh, char *err, size_t err_cap) 
             if (v < kTagPrime) { key[k] = v; break; }
         }
     }
-    if (!sh->store.p) HIP_TRY(sh->store.alloc(g * n * 8));
+    if (!sh->store.p) HIP_TRY(sh->store.alloc(g * n * sizeof(Pair)));
     if (!sh->tag_key.p) HIP_TRY(sh->tag_key.alloc(n * 8));
     if (!sh->tags.p) HIP_TRY(sh->tags.alloc(g * 8));
     if (!ctx->col_scratch.p) HIP_TRY(ctx->col_scratch.alloc(g * n * 2));
@@ -187,11 +189,11 @@ int build_store(inspire_ctx *ctx, inspire_shard *sh, char *err, size_t err_cap) 
     int rc = sync_table(ctx, err, err_cap);
     if (rc) return rc;
     HIP_TRY(launch_store_build(ctx->dev(), sh->records.as<uint16_t>(), ctx->col_scratch.as<uint16_t>(),
-                               sh->tag_key.as<uint64_t>(), sh->store.as<uint64_t>(), sh->tags.as<uint64_t>(), 0));
+                               sh->tag_key.as<uint64_t>(), sh->store.as<Pair>(), sh->tags.as<uint64_t>(), 0));
     HIP_TRY(hipDeviceSynchronize());
     sh->has_store = true;
     ShardDev &d = ctx->shards[sh->id];
-    d.store = sh->store.as<uint64_t>();
+    d.store = sh->store.as<Pair>();
     d.tag_key = sh->tag_key.as<uint64_t>();
     d.tags = sh->tags.as<uint64_t>();
     ctx->table_dirty = true;
@@ -312,9 +314,15 @@ int inspire_ctx_create(const inspire_params *params, const uint8_t w_seed[32], u
     uint64_t inv_b = inv_mod(base), inv_b2 = (uint64_t)((u128)inv_b * inv_b % kQ);
     ctx->inv_n = inv_mod(n);
     ctx->inv_n_gamma = inv_mod((uint64_t)((u128)n * num_columns % kQ));
-    ctx->kx = (uint64_t)((u128)n * inv_b2 % kQ);
-    ctx->k0 = kQ - inv_b2;
-    ctx->k1 = kQ - inv_b;
+    auto pair = [](uint64_t w) { return Pair{w, shoup_companion(w)}; };
+    ctx->kx = pair((uint64_t)((u128)n * inv_b2 % kQ));
+    ctx->k0 = pair(kQ - inv_b2);
+    ctx->k1 = pair(kQ - inv_b);
+    std::vector<Pair> fwd_p(n), inv_p(n);
+    for (uint32_t m = 0; m < n; m++) {
+        fwd_p[m] = pair(fwd[m]);
+        inv_p[m] = pair(inv[m]);
+    }
 
     hipError_t e = hipSuccess;
     DevBuf d_seed;
@@ -322,17 +330,21 @@ int inspire_ctx_create(const inspire_params *params, const uint8_t w_seed[32], u
     if (e == hipSuccess) e = ctx->psi_inv.alloc(n * 8);
     if (e == hipSuccess) e = ctx->gpow.alloc(num_columns * 4);
     if (e == hipSuccess) e = ctx->gpow_back.alloc(num_columns * 4);
-    if (e == hipSuccess) e = ctx->wprime.alloc((size_t)ctx->rows * n * 8);
+    if (e == hipSuccess) e = ctx->twf.alloc(n * sizeof(Pair));
+    if (e == hipSuccess) e = ctx->twi.alloc(n * sizeof(Pair));
+    if (e == hipSuccess) e = ctx->wprime.alloc((size_t)ctx->rows * n * sizeof(Pair));
     if (e == hipSuccess) e = d_seed.alloc(32);
     if (e == hipSuccess) e = hipMemcpy(ctx->psi.p, fwd.data(), n * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(ctx->psi_inv.p, inv.data(), n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ctx->twf.p, fwd_p.data(), n * sizeof(Pair), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ctx->twi.p, inv_p.data(), n * sizeof(Pair), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(ctx->gpow.p, gpow.data(), num_columns * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(ctx->gpow_back.p, gback.data(), num_columns * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_seed.p, w_seed, 32, hipMemcpyHostToDevice);
     // kept_mask_rows (inspiring2.rs:839-855): stream 0 for the exact gadget, gamma otherwise
     if (e == hipSuccess)
         e = launch_expand_mask(ctx->logn, ctx->rows, d_seed.as<uint8_t>(), ctx->dropped ? num_columns : 0,
-                               ctx->dropped, ctx->psi.as<uint64_t>(), ctx->inv_n, ctx->wprime.as<uint64_t>(), 0);
+                               ctx->dropped, ctx->psi.as<uint64_t>(), ctx->inv_n, ctx->wprime.as<Pair>(), 0);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         set_err(err, err_cap, "HIP failure creating the context: %s", hipGetErrorString(e));
@@ -481,12 +493,12 @@ int inspire_seat_create(inspire_ctx *ctx, const uint64_t *y_body, uint32_t rows,
     seat->rows = rows;
     seat->num_to_pack = num_to_pack;
     DevBuf tmp;
-    hipError_t e = seat->yprime.alloc(rows * n * 8);
+    hipError_t e = seat->yprime.alloc(rows * n * sizeof(Pair));
     if (e == hipSuccess) e = tmp.alloc(rows * n * 8);
     if (e == hipSuccess) e = hipMemcpy(tmp.p, y_body, rows * n * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = launch_seat(ctx->logn, (int)rows, tmp.as<uint64_t>(), ctx->psi.as<uint64_t>(), ctx->inv_n,
-                        seat->yprime.as<uint64_t>(), 0);
+                        seat->yprime.as<Pair>(), 0);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         set_err(err, err_cap, "HIP failure seating the packing key: %s", hipGetErrorString(e));
@@ -549,7 +561,7 @@ static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_wo
     b.seeds = d_seeds;
     b.a_in = nullptr;
     b.b = d_b;
-    b.yprime = seat->yprime.as<uint64_t>();
+    b.yprime = seat->yprime.as<Pair>();
     b.scan_part = ws->scan_part.as<uint64_t>();
     b.ks = ctx->ks();
     b.a_out = d_a_out;
@@ -663,7 +675,7 @@ static int respond_host(inspire_ctx *ctx, const inspire_seat *seat, const uint32
     bd.seeds = ws->seeds.as<uint8_t>();
     bd.a_in = a_in ? ws->a_in.as<uint64_t>() : nullptr;
     bd.b = ws->b.as<uint64_t>();
-    bd.yprime = seat->yprime.as<uint64_t>();
+    bd.yprime = seat->yprime.as<Pair>();
     bd.scan_part = ws->scan_part.as<uint64_t>();
     bd.ks = ctx->ks();
     bd.a_out = ws->a_out.as<uint64_t>();

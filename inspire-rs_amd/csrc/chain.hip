@@ -1,0 +1,425 @@
+// The per-query packing chain on gfx950, one workgroup of n/4 threads per query.
+//
+// Restates, fused into one kernel:
+//   CheckedColumns::packing_offline  (src/inspiring/shard_store.rs:282-342)
+//       step 1: R_i = NTT(a)[tab_i] * S_i
+//   fold_rotations_backward          (src/inspiring/inspiring2.rs:1014-1085)
+//       for i = gamma-2 .. 0: T = G^-1(iNTT(R_{i+1})), R_i += sum_k NTT(T_k) * w_all[i][k]
+//       (the last digit's transform read off the others, LastDigitNtt :1094-1161)
+//   packing_online_gather            (src/inspiring/inspiring2.rs:1393-1485)
+//       sum += sum_k y_k[tab_i] * NTT(T_k), fused into the same step
+//   a_hat = iNTT(R_0), b = b_poly + iNTT(sum), then mod_switch_poly (mod_switch.rs:148-175)
+//
+// Arithmetic: every product on the chain has one operand fixed ahead of the query
+// (twiddles, the store's S_i, the mask rows w, the key rows y, the last-digit
+// constants), so all of them are Shoup products a*w - floor(a*w'/2^64)*q with the
+// precomputed companion w' = floor(w*2^64/q). Residues stay lazy in [0, 2q) and are
+// made canonical only where the reference's bytes depend on it (the digits and the
+// outputs). The inverse transform carries no n^-1: S_i, w and y hold it already.
+//
+// Layout: thread t owns slots 4t..4t+3. The first inverse pass (bits 0-1) and the
+// last forward pass (bits 1-0) work on exactly those slots, so the recursion state
+// R and the online sum stay in registers; the passes over higher bits exchange
+// through LDS. The pass over the top bit of the inverse transform, the digit split
+// and the first pass of both forward transforms share one register round.
+#include <hip/hip_runtime.h>
+
+#include "chacha.hpp"
+#include "device.hpp"
+#include "kernels.hpp"
+#include "modq.hpp"
+#include "ntt.hpp"
+
+namespace inspire {
+
+__device__ __forceinline__ uint64_t shoup(uint64_t a, uint64_t w, uint64_t wp) {
+    uint64_t hi;
+    (void)mul_wide(a, wp, hi);
+    return a * w - hi * kQ;  // in [0, 2q) for any a < 2^64
+}
+__device__ __forceinline__ uint64_t shoup(uint64_t a, const Pair &p) { return shoup(a, p.w, p.wp); }
+
+__device__ __forceinline__ uint64_t csub2q(uint64_t x) { return x >= kTwoQ ? x - kTwoQ : x; }
+__device__ __forceinline__ uint64_t csubq(uint64_t x) { return x >= kQ ? x - kQ : x; }
+
+// Gentleman-Sande butterfly on [0, 2q) residues.
+__device__ __forceinline__ void gs(uint64_t &x, uint64_t &y, const Pair &w) {
+    uint64_t s = csub2q(x + y);
+    uint64_t d = x + kTwoQ - y;
+    x = s;
+    y = shoup(d, w);
+}
+// Cooley-Tukey butterfly on [0, 2q) residues.
+__device__ __forceinline__ void ct(uint64_t &x, uint64_t &y, const Pair &w) {
+    uint64_t t = shoup(y, w);
+    uint64_t s = csub2q(x + t);
+    y = csub2q(x + kTwoQ - t);
+    x = s;
+}
+
+// Radix-4 inverse pass over bits (b, b+1) on a group's registers v[k] = x[base + (k << b)].
+template <int LOGN>
+__device__ __forceinline__ void gs4(uint64_t v[4], int b, int hi, const Pair *__restrict__ tw) {
+    constexpr int N = 1 << LOGN;
+    const int m0 = N >> (b + 1), m1 = N >> (b + 2);
+    const Pair w0 = tw[m0 + 2 * hi], w1 = tw[m0 + 2 * hi + 1], w2 = tw[m1 + hi];
+    gs(v[0], v[1], w0);
+    gs(v[2], v[3], w1);
+    gs(v[0], v[2], w2);
+    gs(v[1], v[3], w2);
+}
+// Radix-4 forward pass over bits (b+1, b).
+template <int LOGN>
+__device__ __forceinline__ void ct4(uint64_t v[4], int b, int hi, const Pair *__restrict__ tw) {
+    constexpr int N = 1 << LOGN;
+    const int m0 = N >> (b + 2);
+    const Pair w0 = tw[m0 + hi], w1 = tw[2 * m0 + 2 * hi], w2 = tw[2 * m0 + 2 * hi + 1];
+    ct(v[0], v[2], w0);
+    ct(v[1], v[3], w0);
+    ct(v[0], v[1], w1);
+    ct(v[2], v[3], w2);
+}
+
+__device__ __forceinline__ void group_of(int g, int b, int &base, int &hi) {
+    const int lo = g & ((1 << b) - 1);
+    hi = g >> b;
+    base = (hi << (b + 2)) | lo;
+}
+
+template <int LOGN>
+__device__ __forceinline__ void lds_inv_pass(uint64_t *x, int b, const Pair *tw) {
+    int base, hi;
+    group_of(threadIdx.x, b, base, hi);
+    uint64_t v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = x[lds_ix(base + (k << b))];
+    gs4<LOGN>(v, b, hi, tw);
+#pragma unroll
+    for (int k = 0; k < 4; k++) x[lds_ix(base + (k << b))] = v[k];
+}
+
+template <int LOGN>
+__device__ __forceinline__ void lds_fwd_pass2(uint64_t *x, uint64_t *y, int b, const Pair *tw) {
+    int base, hi;
+    group_of(threadIdx.x, b, base, hi);
+    uint64_t u[4], v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        u[k] = x[lds_ix(base + (k << b))];
+        v[k] = y[lds_ix(base + (k << b))];
+    }
+    ct4<LOGN>(u, b, hi, tw);
+    ct4<LOGN>(v, b, hi, tw);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        x[lds_ix(base + (k << b))] = u[k];
+        y[lds_ix(base + (k << b))] = v[k];
+    }
+}
+
+// Inverse transform of the register state s (own slots) into LDS x, up to and not
+// including the passes that the caller fuses. Leaves x holding the values after the
+// inverse passes over bits [0, stop), with a barrier at the end.
+template <int LOGN>
+__device__ __forceinline__ void inverse_to(uint64_t s[4], uint64_t *x, int stop, const Pair *tw) {
+    uint64_t v[4] = {s[0], s[1], s[2], s[3]};
+    gs4<LOGN>(v, 0, threadIdx.x, tw);
+    const int base = 4 * threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 4; k++) x[lds_ix(base + k)] = v[k];
+    __syncthreads();
+    for (int b = 2; b < stop; b += 2) {
+        lds_inv_pass<LOGN>(x, b, tw);
+        __syncthreads();
+    }
+}
+
+// digits of a canonical coefficient c (gadget_decompose, external_product.rs:65-99;
+// rounded form for a dropped lowest digit, inspiring2.rs:1184-1225)
+template <bool EXACT>
+__device__ __forceinline__ void digits(uint64_t c, uint64_t &d0, uint64_t &d1) {
+    if (EXACT) {
+        d0 = c & 0xfffff;
+        d1 = (c >> 20) & 0xfffff;
+    } else {
+        uint64_t r = (c + (1ULL << 19)) >> 20;
+        d0 = r & 0xfffff;
+        d1 = r >> 20;
+    }
+}
+
+// The top of the inverse transform, the digit split and the top of both forward
+// transforms for this thread's share, then the remaining forward passes down to
+// bits 3-2. On return C and D hold the digit transforms after all passes but the
+// last (bits 1-0), barrier included.
+template <int LOGN, bool EXACT>
+__device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi, const Pair *twf) {
+    constexpr int N = 1 << LOGN;
+    if constexpr (LOGN & 1) {
+        // single top bit: pairs (e, e + N/2), two per thread
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            const int e = threadIdx.x + r * (N / 4);
+            uint64_t x = C[lds_ix(e)], y = C[lds_ix(e + N / 2)];
+            gs(x, y, twi[1]);
+            uint64_t a0, a1, b0, b1;
+            digits<EXACT>(csubq(x), a0, a1);
+            digits<EXACT>(csubq(y), b0, b1);
+            ct(a0, b0, twf[1]);
+            ct(a1, b1, twf[1]);
+            C[lds_ix(e)] = a0;
+            C[lds_ix(e + N / 2)] = b0;
+            D[lds_ix(e)] = a1;
+            D[lds_ix(e + N / 2)] = b1;
+        }
+        __syncthreads();
+        for (int b = LOGN - 3; b >= 2; b -= 2) {
+            lds_fwd_pass2<LOGN>(C, D, b, twf);
+            __syncthreads();
+        }
+    } else {
+        // top two bits: one radix-4 group per thread
+        int base, hi;
+        const int b = LOGN - 2;
+        group_of(threadIdx.x, b, base, hi);
+        uint64_t v[4], u[4], w[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = C[lds_ix(base + (k << b))];
+        gs4<LOGN>(v, b, hi, twi);
+#pragma unroll
+        for (int k = 0; k < 4; k++) digits<EXACT>(csubq(v[k]), u[k], w[k]);
+        ct4<LOGN>(u, b, hi, twf);
+        ct4<LOGN>(w, b, hi, twf);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            C[lds_ix(base + (k << b))] = u[k];
+            D[lds_ix(base + (k << b))] = w[k];
+        }
+        __syncthreads();
+        for (int bb = LOGN - 4; bb >= 2; bb -= 2) {
+            lds_fwd_pass2<LOGN>(C, D, bb, twf);
+            __syncthreads();
+        }
+    }
+}
+
+template <int LOGN, int ROWS, bool EXACT>
+__global__ __launch_bounds__((1 << LOGN) / 4) void k_chain(CtxDev c, BatchDev bt) {
+    constexpr int N = 1 << LOGN;
+    constexpr int T = N / 4;
+    __shared__ uint64_t A[LdsSize<LOGN>::value];  // NTT(a)
+    __shared__ uint64_t C[LdsSize<LOGN>::value];  // transform buffer / digit 0
+    __shared__ uint64_t D[LdsSize<LOGN>::value];  // digit 1
+    __shared__ uint64_t bpoly[1024];
+    __shared__ int bad_column;
+    __shared__ int rejected;
+
+    const int qi = blockIdx.x, tid = threadIdx.x, gamma = c.gamma;
+    const ShardDev sh = c.shards[bt.shard_ids[qi]];
+    const Pair *twf = c.twf, *twi = c.twi;
+
+    // --- finish the scan: b_poly and the column tags, before any packing work
+    if (tid == 0) { bad_column = 0x7fffffff; rejected = 0; }
+    __syncthreads();
+    for (int j = tid; j < gamma; j += T) {
+        uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+        for (int s = 0; s < bt.ks; s++) {
+            const uint64_t *p = bt.scan_part + (((size_t)qi * bt.ks + s) * gamma + j) * 4;
+            s0 += p[0]; s1 += p[1]; s2 += p[2]; s3 += p[3];
+        }
+        uint64_t lo = s0 + (s1 << 32), hi = (s1 >> 32) + (lo < s0);
+        bpoly[j] = reduce120(hi, lo);
+        uint64_t tlo = s2 + (s3 << 32), thi = (s3 >> 32) + (tlo < s2);
+        if (mod_p61(thi, tlo) != sh.tags[j]) atomicMin(&bad_column, j);
+    }
+    __syncthreads();
+    if (bad_column != 0x7fffffff) {
+        if (tid == 0) bt.status[qi] = bad_column + 1;
+        return;
+    }
+
+    // --- a: the query's expanded row, or gen_range(0..q) draws from ChaCha20(seed)
+    if (bt.a_in) {
+        for (int j = tid; j < N; j += T) A[lds_ix(j)] = bt.a_in[(size_t)qi * N + j];
+    } else {
+        uint32_t key[8];
+        seed_key(bt.seeds + (size_t)qi * 32, key);
+        for (int blk = tid; blk < N / 8; blk += T) {
+            uint32_t w[16];
+            chacha20_block(key, (uint64_t)blk, 0, w);
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                uint64_t v = (uint64_t)w[2 * u] | (uint64_t)w[2 * u + 1] << 32, a;
+                if (!uniform_q_draw(v, a)) rejected = 1;
+                A[lds_ix(blk * 8 + u)] = a;
+            }
+        }
+        __syncthreads();
+        if (rejected) {
+            // a draw outside the acceptance zone (probability ~2^-46 each): replay the
+            // stream serially with rejection, as the reference's RNG does
+            if (tid == 0) {
+                uint64_t blk = 0;
+                int word = 16, got = 0;
+                uint32_t w[16];
+                while (got < N) {
+                    uint32_t lo32, hi32;
+                    if (word >= 16) { chacha20_block(key, blk++, 0, w); word = 0; }
+                    lo32 = w[word++];
+                    if (word >= 16) { chacha20_block(key, blk++, 0, w); word = 0; }
+                    hi32 = w[word++];
+                    uint64_t a;
+                    if (uniform_q_draw((uint64_t)lo32 | (uint64_t)hi32 << 32, a)) A[lds_ix(got++)] = a;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // NTT(a) with the radix-4 forward passes from the top bit down, in place in A
+    {
+        if constexpr (LOGN & 1) {
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                const int e = tid + r * (N / 4);
+                uint64_t x = A[lds_ix(e)], y = A[lds_ix(e + N / 2)];
+                ct(x, y, twf[1]);
+                A[lds_ix(e)] = x;
+                A[lds_ix(e + N / 2)] = y;
+            }
+            __syncthreads();
+        }
+        for (int b = (LOGN & 1) ? LOGN - 3 : LOGN - 2; b >= 0; b -= 2) {
+            int base, hi;
+            group_of(tid, b, base, hi);
+            uint64_t v[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) v[k] = A[lds_ix(base + (k << b))];
+            ct4<LOGN>(v, b, hi, twf);
+#pragma unroll
+            for (int k = 0; k < 4; k++) A[lds_ix(base + (k << b))] = v[k];
+            __syncthreads();
+        }
+    }
+
+    // --- step 1 for the last rotation, on own slots
+    uint64_t X[4], acc[4];
+    {
+        const uint32_t t = c.gpow[gamma - 1];
+        const Pair *S = sh.store + (size_t)(gamma - 1) * N;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int j = 4 * tid + k;
+            X[k] = shoup(A[lds_ix(automorph_slot(t, j, LOGN))], S[j]);
+            acc[k] = 0;
+        }
+    }
+
+    // --- backward recursion with the online gather
+    for (int i = gamma - 2; i >= 0; i--) {
+        inverse_to<LOGN>(X, C, (LOGN & 1) ? LOGN - 1 : LOGN - 2, twi);
+        middle<LOGN, EXACT>(C, D, twi, twf);
+        // last forward pass (bits 1-0) on own slots, then the pointwise work
+        uint64_t t0[4], t1[4];
+        const int base = 4 * tid;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            t0[k] = C[lds_ix(base + k)];
+            t1[k] = D[lds_ix(base + k)];
+        }
+        ct4<LOGN>(t0, 0, tid, twf);
+        ct4<LOGN>(t1, 0, tid, twf);
+        const uint32_t t = c.gpow[i];
+        const Pair *S = sh.store + (size_t)i * N;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int j = base + k;
+            const uint32_t src = automorph_slot(t, j, LOGN);
+            uint64_t d[3] = {t0[k], t1[k], 0};
+            if constexpr (ROWS == 3) d[2] = shoup(X[k], c.kx) + shoup(d[0], c.k0) + shoup(d[1], c.k1);  // < 6q
+            uint64_t r = shoup(A[lds_ix(src)], S[j]);
+            uint64_t o = acc[k];
+#pragma unroll
+            for (int rr = 0; rr < ROWS; rr++) {
+                r += shoup(d[rr], c.wprime[(size_t)rr * N + src]);
+                o += shoup(d[rr], bt.yprime[(size_t)rr * N + src]);
+            }
+            // r < (1 + ROWS) * 2q <= 8q, o < (1 + ROWS) * 2q: back to [0, 2q)
+            r = r >= 4 * kQ ? r - 4 * kQ : r;
+            o = o >= 4 * kQ ? o - 4 * kQ : o;
+            X[k] = csub2q(r);
+            acc[k] = csub2q(o);
+        }
+    }
+
+    // --- a_hat = iNTT(R_0) into C, iNTT(sum) into D
+    {
+        uint64_t v[4] = {X[0], X[1], X[2], X[3]}, u[4] = {acc[0], acc[1], acc[2], acc[3]};
+        gs4<LOGN>(v, 0, tid, twi);
+        gs4<LOGN>(u, 0, tid, twi);
+        const int base = 4 * tid;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            C[lds_ix(base + k)] = v[k];
+            D[lds_ix(base + k)] = u[k];
+        }
+        __syncthreads();
+        for (int b = 2; b < LOGN - 1; b += 2) {
+            lds_inv_pass<LOGN>(C, b, twi);
+            lds_inv_pass<LOGN>(D, b, twi);
+            __syncthreads();
+        }
+        if constexpr (LOGN & 1) {
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                const int e = tid + r * (N / 4);
+                uint64_t x = C[lds_ix(e)], y = C[lds_ix(e + N / 2)];
+                gs(x, y, twi[1]);
+                C[lds_ix(e)] = csubq(x);
+                C[lds_ix(e + N / 2)] = csubq(y);
+                x = D[lds_ix(e)], y = D[lds_ix(e + N / 2)];
+                gs(x, y, twi[1]);
+                D[lds_ix(e)] = csubq(x);
+                D[lds_ix(e + N / 2)] = csubq(y);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int e = tid + r * T;
+                C[lds_ix(e)] = csubq(C[lds_ix(e)]);
+                D[lds_ix(e)] = csubq(D[lds_ix(e)]);
+            }
+        }
+        __syncthreads();
+    }
+    const uint64_t qn = bt.switch_modulus;
+    for (int j = tid; j < N; j += T) {
+        uint64_t a = C[lds_ix(j)];
+        bt.a_out[(size_t)qi * N + j] = qn ? mod_switch_coeff(a, qn) : a;
+    }
+    for (int j = tid; j < gamma; j += T) {
+        uint64_t bv = addmod(bpoly[j], D[lds_ix(j)]);
+        bt.b_out[(size_t)qi * gamma + j] = qn ? mod_switch_coeff(bv, qn) : bv;
+    }
+    if (tid == 0) bt.status[qi] = 0;
+}
+
+template <int LOGN>
+static hipError_t launch_chain_t(const CtxDev &c, const BatchDev &b, hipStream_t s) {
+    constexpr int T = (1 << LOGN) / 4;
+    if (c.rows == 3)
+        hipLaunchKernelGGL((k_chain<LOGN, 3, true>), dim3(b.count), dim3(T), 0, s, c, b);
+    else
+        hipLaunchKernelGGL((k_chain<LOGN, 2, false>), dim3(b.count), dim3(T), 0, s, c, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_chain_only(const CtxDev &c, const BatchDev &b, hipStream_t s) {
+    if (b.count == 0) return hipSuccess;
+    switch (c.logn) {
+        case 8: return launch_chain_t<8>(c, b, s);
+        case 11: return launch_chain_t<11>(c, b, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace inspire

@@ -4,11 +4,16 @@
 
 namespace inspire {
 
+// A fixed multiplicand w < q with its Shoup companion wp = floor(w * 2^64 / q).
+struct Pair {
+    uint64_t w, wp;
+};
+
 // One shard resident in HBM (the reference's ShardData + ShardPackingStore,
 // src/pir/setup.rs:280-289, src/inspiring/shard_store.rs:36-48).
 struct ShardDev {
     const uint16_t *records;  // ring_dim x gamma, row-major: column j of record m at [m*gamma + j]
-    const uint64_t *store;    // gamma x ring_dim: n^-1 * S_i in NTT form (standard residues)
+    const Pair *store;        // gamma x ring_dim: n^-1 * S_i in NTT form, with companions
     const uint64_t *tag_key;  // ring_dim tag key words, each < 2^61 - 1
     const uint64_t *tags;     // gamma column tags mod 2^61 - 1
 };
@@ -19,11 +24,13 @@ struct CtxDev {
     int logn, n, gamma, rows, exact;
     const uint64_t *psi;        // n, forward ladder psi^brv(m), standard form
     const uint64_t *psi_inv;    // n, inverse ladder
-    const uint64_t *wprime;     // rows x n: n^-1 * NTT(w_mask_k)
+    const Pair *twf;            // n, forward ladder with companions
+    const Pair *twi;            // n, inverse ladder with companions
+    const Pair *wprime;         // rows x n: n^-1 * NTT(w_mask_k)
     const uint32_t *gpow;       // gamma: g^i mod 2n
     const uint32_t *gpow_back;  // gamma: g^(n-i mod n) mod 2n
     uint64_t inv_n_gamma;       // (n * gamma)^-1 mod q
-    uint64_t kx, k0, k1;        // last-digit constants (exact gadget)
+    Pair kx, k0, k1;            // last-digit constants (exact gadget)
     const ShardDev *shards;     // dense shard table
     int num_shards;
 };
@@ -35,7 +42,7 @@ struct BatchDev {
     const uint8_t *seeds;       // count x 32 (seeded queries), or
     const uint64_t *a_in;       // count x n expanded a (unseeded queries) when non-null
     const uint64_t *b;          // count x n
-    const uint64_t *yprime;     // rows x n: n^-1 * NTT(y_body_k) of the seat
+    const Pair *yprime;         // rows x n: n^-1 * NTT(y_body_k) of the seat
     uint64_t *scan_part;        // count x ks x gamma x 4 scratch
     int ks;                     // row splits of the scan
     uint64_t *a_out;            // count x n

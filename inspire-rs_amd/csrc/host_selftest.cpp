@@ -24,5 +24,11 @@ void st_chacha_block(const uint8_t *seed, uint64_t counter, uint64_t stream, uin
     seed_key(seed, key);
     chacha20_block(key, counter, stream, out);
 }
+uint64_t st_shoup_companion(uint64_t w) { return shoup_companion(w); }
+uint64_t st_shoup(uint64_t a, uint64_t w) {
+    uint64_t wp = shoup_companion(w), hi;
+    (void)mul_wide(a, wp, hi);
+    return a * w - hi * kQ;
+}
 int st_uniform_q_draw(uint64_t v, uint64_t *out) { return uniform_q_draw(v, *out) ? 1 : 0; }
 }

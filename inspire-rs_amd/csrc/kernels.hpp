@@ -10,10 +10,10 @@ hipError_t launch_respond(const CtxDev &c, const BatchDev &b, hipStream_t s);
 hipError_t launch_scan_only(const CtxDev &c, const BatchDev &b, hipStream_t s);
 hipError_t launch_chain_only(const CtxDev &c, const BatchDev &b, hipStream_t s);
 hipError_t launch_expand_mask(int logn, int rows, const uint8_t *seed, uint64_t stream, int dropped,
-                              const uint64_t *psi, uint64_t inv_n, uint64_t *out, hipStream_t s);
-hipError_t launch_seat(int logn, int rows, const uint64_t *y, const uint64_t *psi, uint64_t inv_n, uint64_t *out,
+                              const uint64_t *psi, uint64_t inv_n, Pair *out, hipStream_t s);
+hipError_t launch_seat(int logn, int rows, const uint64_t *y, const uint64_t *psi, uint64_t inv_n, Pair *out,
                        hipStream_t s);
 hipError_t launch_store_build(const CtxDev &c, const uint16_t *records, uint16_t *col_scratch, const uint64_t *tag_key,
-                              uint64_t *store, uint64_t *tags, hipStream_t s);
+                              Pair *store, uint64_t *tags, hipStream_t s);
 
 }  // namespace inspire

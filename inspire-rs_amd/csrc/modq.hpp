@@ -118,6 +118,23 @@ __host__ __device__ __forceinline__ uint64_t mod_switch_coeff(uint64_t c, uint64
     return quot >= qn ? quot - qn : quot;
 }
 
+// Shoup companion floor(w * 2^64 / q) of a residue w < q. Estimate from
+// 2^64/q = 16 (1 + d/2^60 + ...), d = 2^14 - 1, then correct against the exact remainder.
+__host__ __device__ __forceinline__ uint64_t shoup_companion(uint64_t w) {
+    uint64_t eh, el = mul_wide(w, (1ULL << 14) - 1, eh);
+    uint64_t e = (w << 4) + ((eh << 8) | (el >> 56));
+    for (int it = 0; it < 4; it++) {
+        // r = w*2^64 - e*q as a signed 128-bit value (r_hi, r_lo)
+        uint64_t ph, pl = mul_wide(e, kQ, ph);
+        uint64_t r_lo = 0 - pl;
+        uint64_t r_hi = w - ph - (pl != 0 ? 1 : 0);
+        if ((int64_t)r_hi < 0) { e--; continue; }
+        if (r_hi == 0 && r_lo < kQ) break;
+        e++;
+    }
+    return e;
+}
+
 __host__ __device__ __forceinline__ uint32_t bitrev(uint32_t x, int bits) {
 #ifdef __HIP_DEVICE_COMPILE__
     return __builtin_bitreverse32(x) >> (32 - bits);

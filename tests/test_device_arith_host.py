@@ -35,6 +35,10 @@ def st():
     L.st_automorph_slot.argtypes = [u32, u32, ctypes.c_int]
     L.st_automorph_slot.restype = u32
     L.st_chacha_block.argtypes = [ctypes.c_char_p, u64, u64, ctypes.c_void_p]
+    L.st_shoup_companion.argtypes = [u64]
+    L.st_shoup_companion.restype = u64
+    L.st_shoup.argtypes = [u64, u64]
+    L.st_shoup.restype = u64
     L.st_uniform_q_draw.argtypes = [u64, ctypes.POINTER(u64)]
     return L
 
@@ -126,3 +130,16 @@ def test_chacha_and_uniform_draws_match_oracle(st):
     assert st.st_uniform_q_draw(v_reject, ctypes.byref(r)) == 0
     v_accept = (zone * qinv) % 2**64
     assert st.st_uniform_q_draw(v_accept, ctypes.byref(r)) == 1
+
+
+def test_shoup_companion_and_product_range(st):
+    rng = np.random.default_rng(6)
+    ws = EDGES + [int(x) for x in rng.integers(0, Q, 500, dtype=np.uint64)]
+    for w in ws:
+        w %= Q
+        assert st.st_shoup_companion(w) == (w << 64) // Q
+    # a*w - floor(a*w'/2^64)*q lies in [0, 2q) for every a < 2^64 and is = a*w (mod q)
+    for a in [0, 1, Q - 1, 2 * Q - 1, 4 * Q - 1, 2**64 - 1] + [int(x) for x in rng.integers(0, 2**63, 300, dtype=np.uint64)]:
+        w = int(rng.integers(0, Q, dtype=np.uint64))
+        r = st.st_shoup(a, w)
+        assert r < 2 * Q and r % Q == a * w % Q
