@@ -75,14 +75,19 @@ def cpu_baseline(budget_s=10.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=48)
+    ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--inflight", type=int, default=4, help="batches in flight on separate streams")
+    ap.add_argument("--queries-per-cu", type=int, default=1, help="launch shaping of the packing kernel (1 or 2)")
     ap.add_argument("--shards", type=int, default=128)
     ap.add_argument("--plain-shards", type=int, default=32)
     ap.add_argument("--plain-queries", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy a 256 MiB device buffer once: a kernel of known HBM traffic that "
+                         "scales FETCH_SIZE / WRITE_SIZE counters read from the same profiled run")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the respond path shards by shard id with no exchange, "
@@ -98,58 +103,85 @@ def main():
     rng = np.random.default_rng(12345)
     w_seed = rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
 
+    if args.calibrate_copy:
+        src = torch.empty(256 << 20, dtype=torch.uint8, device=dev).fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     log(f"bench: building {args.shards} shards of {N} x {ENTRY} B with stores")
     ctx = g.Context(g.params_d2048(), w_seed, GAMMA)
+    ctx.set_queries_per_cu(args.queries_per_cu)
     t0 = time.perf_counter()
     build_db(g, ctx, args.shards, seed=1)
     build_s = time.perf_counter() - t0
     y = rng.integers(0, g.Q, (3, N), dtype=np.uint64)
     seat = ctx.seat(y)
     B = args.batch
-    ws = g.Workspace(ctx, B)
+    K = args.inflight
+    streams = [torch.cuda.Stream() for _ in range(K)]  # default priority
+    wss = [g.Workspace(ctx, B) for _ in range(K)]
 
+    args.warmup = max(args.warmup, K)
     total = args.warmup + args.steps
     ids, seeds, bs = make_batches(rng, total, B, args.shards, g.Q)
     d_ids = torch.from_numpy(ids.astype(np.int32)).to(dev)
     d_seeds = torch.from_numpy(seeds).to(dev)
     d_b = torch.from_numpy(bs.view(np.int64)).to(dev)
-    d_a_out = torch.empty((B, N), dtype=torch.int64, device=dev)
-    d_b_out = torch.empty((B, GAMMA), dtype=torch.int64, device=dev)
+    d_a_out = [torch.empty((B, N), dtype=torch.int64, device=dev) for _ in range(K)]
+    d_b_out = [torch.empty((B, GAMMA), dtype=torch.int64, device=dev) for _ in range(K)]
+    torch.cuda.synchronize()
 
-    def step(i, timed=False):
-        args_ = (ctx, seat, ws, d_ids[i].data_ptr(), d_seeds[i].data_ptr(), d_b[i].data_ptr(), B,
-                 d_a_out.data_ptr(), d_b_out.data_ptr())
-        if timed:
-            return g.respond_batch_device_timed(*args_, stream=stream, switch_modulus=Q36)
-        ctx.respond_batch_device(*args_[1:], stream=stream, switch_modulus=Q36)
+    def issue(i, slot):
+        st = streams[slot]
+        ctx.respond_batch_device(seat, wss[slot], d_ids[i].data_ptr(), d_seeds[i].data_ptr(), d_b[i].data_ptr(), B,
+                                 d_a_out[slot].data_ptr(), d_b_out[slot].data_ptr(), stream=st.cuda_stream,
+                                 switch_modulus=Q36)
 
     log("bench: warmup")
+    assert args.warmup >= K, "warmup must cover every stream slot once"
     for i in range(args.warmup):
-        step(i)
+        issue(i, i % K)
     torch.cuda.synchronize()
-    st = ws.status(B)
-    assert (st == 0).all(), f"respond status {st}"
+    for w in wss:
+        st_ = w.status(B)
+        assert (st_ == 0).all(), f"respond status {st_}"
 
-    # timed region: K steps, per-step latency from events on the launch stream
+    # timed region: `steps` batches issued round-robin over K streams (a stream runs its
+    # batches in order, so a slot's buffers are reused only after its previous batch);
+    # per-batch latency from events on each batch's own stream
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        starts[k].record()
-        step(args.warmup + k)
-        ends[k].record()
+        slot = k % K
+        st = streams[slot]
+        starts[k].record(st)
+        issue(args.warmup + k, slot)
+        ends[k].record(st)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    lat = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])
+    lat = np.array([s_.elapsed_time(e) for s_, e in zip(starts, ends)])
     ms_per_step = wall * 1000 / args.steps
     qps = B * args.steps / wall
+    for w in wss:
+        st_ = w.status(B)
+        assert (st_ == 0).all(), f"respond status {st_}"
+    ws = wss[0]
+
+    def step(i, timed=False):
+        args_ = (ctx, seat, ws, d_ids[i].data_ptr(), d_seeds[i].data_ptr(), d_b[i].data_ptr(), B,
+                 d_a_out[0].data_ptr(), d_b_out[0].data_ptr())
+        return g.respond_batch_device_timed(*args_, stream=stream, switch_modulus=Q36)
 
     # per-kernel durations (HIP events on the launch stream around each kernel)
     log("bench: per-kernel timing")
     scan_ms, chain_ms = [], []
     for k in range(min(args.steps, 5)):
-        s_ms, c_ms = step(args.warmup + k, timed=True)
+        s_ms, c_ms = step(args.warmup + k)
         scan_ms.append(s_ms)
         chain_ms.append(c_ms)
     scan_avg, chain_avg = float(np.mean(scan_ms)), float(np.mean(chain_ms))
@@ -210,7 +242,9 @@ def main():
         "config": {
             "workload": f"batched TwoPacking respond, d2048, 2^18 records x 512 B ({args.shards} shards, gamma 256), "
                         f"stores precomputed, {B} seeded queries per batch under one seat, uniform indices, "
-                        "36-bit modulus switch",
+                        f"36-bit modulus switch, {K} batches in flight on {K} streams",
+            "inflight": K,
+            "queries_per_cu": args.queries_per_cu,
             "global_batch": B,
             "records": args.shards * N,
             "record_bytes": ENTRY,

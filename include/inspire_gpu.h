@@ -56,6 +56,10 @@ void inspire_ctx_destroy(inspire_ctx *ctx);
 /* key rows the seats must carry: packing_gadget_len minus the dropped digits
  * (PackParams::key_rows, src/inspiring/inspiring2.rs:289-291) */
 uint32_t inspire_ctx_key_rows(const inspire_ctx *ctx);
+/* Launch shaping of the packing kernel (no reference counterpart; results are
+ * unchanged): 1 (default) gives every query a CU of its own, 2 lets two queries share
+ * a CU, which raises throughput when far more queries are in flight than CUs. */
+int inspire_ctx_set_queries_per_cu(inspire_ctx *ctx, uint32_t queries_per_cu, char *err, size_t err_cap);
 
 /* Upload one shard of the EncodedDatabase (ShardData, src/pir/setup.rs:280-289).
  * columns: num_columns x ring_dim column polynomials (ShardData::polynomials), each

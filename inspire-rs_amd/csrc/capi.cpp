@@ -126,9 +126,10 @@ struct inspire_ctx {
     std::vector<ShardDev> shards;        // dense by id
     std::vector<inspire_shard *> owners;
     bool table_dirty = true;
+    int queries_per_cu = 1;
     inspire_workspace *default_ws = nullptr;
 
-    int ks() const { return n >= 2048 ? 8 : 1; }
+    int ks() const { return n >= 2048 ? 16 : 1; }
 
     CtxDev dev() const {
         CtxDev c;
@@ -140,6 +141,7 @@ struct inspire_ctx {
         c.inv_n_gamma = inv_n_gamma; c.kx = kx; c.k0 = k0; c.k1 = k1;
         c.shards = shard_table.as<ShardDev>();
         c.num_shards = (int)shards.size();
+        c.queries_per_cu = queries_per_cu;
         return c;
     }
 };
@@ -345,6 +347,8 @@ int inspire_ctx_create(const inspire_params *params, const uint8_t w_seed[32], u
     if (e == hipSuccess)
         e = launch_expand_mask(ctx->logn, ctx->rows, d_seed.as<uint8_t>(), ctx->dropped ? num_columns : 0,
                                ctx->dropped, ctx->psi.as<uint64_t>(), ctx->inv_n, ctx->wprime.as<Pair>(), 0);
+    if (e == hipSuccess && ctx->rows == 3)
+        e = launch_fold_rows(ctx->wprime.as<Pair>(), (int)n, ctx->kx.w, ctx->k0.w, ctx->k1.w, 0);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         set_err(err, err_cap, "HIP failure creating the context: %s", hipGetErrorString(e));
@@ -362,6 +366,16 @@ void inspire_ctx_destroy(inspire_ctx *ctx) {
 }
 
 uint32_t inspire_ctx_key_rows(const inspire_ctx *ctx) { return ctx ? (uint32_t)ctx->rows : 0; }
+
+int inspire_ctx_set_queries_per_cu(inspire_ctx *ctx, uint32_t queries_per_cu, char *err, size_t err_cap) {
+    if (!ctx || queries_per_cu < 1 || queries_per_cu > 2) {
+        set_err(err, err_cap, "queries per CU must be 1 or 2");
+        return INSPIRE_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->queries_per_cu = (int)queries_per_cu;
+    return INSPIRE_OK;
+}
 
 int inspire_shard_upload_columns(inspire_ctx *ctx, uint32_t shard_id, const uint64_t *columns, uint32_t num_columns,
                                  inspire_shard **out, char *err, size_t err_cap) {
@@ -499,6 +513,8 @@ int inspire_seat_create(inspire_ctx *ctx, const uint64_t *y_body, uint32_t rows,
     if (e == hipSuccess)
         e = launch_seat(ctx->logn, (int)rows, tmp.as<uint64_t>(), ctx->psi.as<uint64_t>(), ctx->inv_n,
                         seat->yprime.as<Pair>(), 0);
+    if (e == hipSuccess && rows == 3)
+        e = launch_fold_rows(seat->yprime.as<Pair>(), (int)n, ctx->kx.w, ctx->k0.w, ctx->k1.w, 0);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         set_err(err, err_cap, "HIP failure seating the packing key: %s", hipGetErrorString(e));

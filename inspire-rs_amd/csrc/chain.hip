@@ -49,12 +49,12 @@ __device__ __forceinline__ void gs(uint64_t &x, uint64_t &y, const Pair &w) {
     x = s;
     y = shoup(d, w);
 }
-// Cooley-Tukey butterfly on [0, 2q) residues.
+// Cooley-Tukey butterfly (Harvey): inputs and outputs in [0, 4q).
 __device__ __forceinline__ void ct(uint64_t &x, uint64_t &y, const Pair &w) {
     uint64_t t = shoup(y, w);
-    uint64_t s = csub2q(x + t);
-    y = csub2q(x + kTwoQ - t);
-    x = s;
+    uint64_t u = csub2q(x);
+    x = u + t;
+    y = u + kTwoQ - t;
 }
 
 // Radix-4 inverse pass over bits (b, b+1) on a group's registers v[k] = x[base + (k << b)].
@@ -196,6 +196,7 @@ __device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi
             D[lds_ix(base + (k << b))] = w[k];
         }
         __syncthreads();
+#pragma unroll
         for (int bb = LOGN - 4; bb >= 2; bb -= 2) {
             lds_fwd_pass2<LOGN>(C, D, bb, twf);
             __syncthreads();
@@ -204,7 +205,7 @@ __device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi
 }
 
 template <int LOGN, int ROWS, bool EXACT>
-__global__ __launch_bounds__((1 << LOGN) / 4) void k_chain(CtxDev c, BatchDev bt) {
+__global__ __launch_bounds__((1 << LOGN) / 4, 4) void k_chain(CtxDev c, BatchDev bt) {
     constexpr int N = 1 << LOGN;
     constexpr int T = N / 4;
     __shared__ uint64_t A[LdsSize<LOGN>::value];  // NTT(a)
@@ -334,8 +335,10 @@ __global__ __launch_bounds__((1 << LOGN) / 4) void k_chain(CtxDev c, BatchDev bt
         for (int k = 0; k < 4; k++) {
             const int j = base + k;
             const uint32_t src = automorph_slot(t, j, LOGN);
-            uint64_t d[3] = {t0[k], t1[k], 0};
-            if constexpr (ROWS == 3) d[2] = shoup(X[k], c.kx) + shoup(d[0], c.k0) + shoup(d[1], c.k1);  // < 6q
+            // exact gadget: the third digit's transform is kx*R + k0*T0 + k1*T1, folded into the
+            // rows (W_2 = kx*w_2, W_0 = w_0 + k0*w_2, W_1 = w_1 + k1*w_2; same for y), so its
+            // operand here is R itself
+            const uint64_t d[3] = {t0[k], t1[k], X[k]};
             uint64_t r = shoup(A[lds_ix(src)], S[j]);
             uint64_t o = acc[k];
 #pragma unroll
@@ -403,13 +406,22 @@ __global__ __launch_bounds__((1 << LOGN) / 4) void k_chain(CtxDev c, BatchDev bt
     if (tid == 0) bt.status[qi] = 0;
 }
 
+// One query per CU unless asked otherwise: a workgroup then reserves more than half
+// of the CU's 160 KiB LDS (dynamic padding, never touched), so the dispatcher spreads
+// a launch over all CUs instead of pairing queries on some and idling others.
+constexpr size_t kLdsPerCu = 160 * 1024;
+
 template <int LOGN>
 static hipError_t launch_chain_t(const CtxDev &c, const BatchDev &b, hipStream_t s) {
     constexpr int T = (1 << LOGN) / 4;
-    if (c.rows == 3)
-        hipLaunchKernelGGL((k_chain<LOGN, 3, true>), dim3(b.count), dim3(T), 0, s, c, b);
-    else
-        hipLaunchKernelGGL((k_chain<LOGN, 2, false>), dim3(b.count), dim3(T), 0, s, c, b);
+    auto kern = c.rows == 3 ? k_chain<LOGN, 3, true> : k_chain<LOGN, 2, false>;
+    hipFuncAttributes attr;
+    hipError_t e = hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(kern));
+    if (e != hipSuccess) return e;
+    size_t dyn = 0;
+    if (c.queries_per_cu <= 1 && attr.sharedSizeBytes <= kLdsPerCu / 2)
+        dyn = kLdsPerCu / 2 + 256 - attr.sharedSizeBytes;
+    hipLaunchKernelGGL(kern, dim3(b.count), dim3(T), dyn, s, c, b);
     return hipGetLastError();
 }
 

@@ -33,6 +33,7 @@ struct CtxDev {
     Pair kx, k0, k1;            // last-digit constants (exact gadget)
     const ShardDev *shards;     // dense shard table
     int num_shards;
+    int queries_per_cu;         // launch shaping: 1 spreads queries over CUs, 2 co-schedules pairs
 };
 
 // Inputs and outputs of one respond batch, device pointers.

@@ -64,6 +64,22 @@ __global__ __launch_bounds__(kThreads) void k_seat(const uint64_t *y, const uint
     }
 }
 
+// Folds the derived last digit into three rows: (r0, r1, r2) <- (r0 + k0 r2, r1 + k1 r2, kx r2).
+__global__ void k_fold_rows(Pair *rows, int n, uint64_t kx, uint64_t k0, uint64_t k1) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint64_t r0 = rows[j].w, r1 = rows[n + j].w, r2 = rows[2 * n + j].w;
+    uint64_t n0 = addmod(r0, mulmod(k0, r2)), n1 = addmod(r1, mulmod(k1, r2)), n2 = mulmod(kx, r2);
+    rows[j] = Pair{n0, shoup_companion(n0)};
+    rows[n + j] = Pair{n1, shoup_companion(n1)};
+    rows[2 * n + j] = Pair{n2, shoup_companion(n2)};
+}
+
+hipError_t launch_fold_rows(Pair *rows, int n, uint64_t kx, uint64_t k0, uint64_t k1, hipStream_t s) {
+    hipLaunchKernelGGL(k_fold_rows, dim3((n + 255) / 256), dim3(256), 0, s, rows, n, kx, k0, k1);
+    return hipGetLastError();
+}
+
 __global__ void k_transpose(const uint16_t *rec, int n, int gamma, uint16_t *col) {
     size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (size_t)n * gamma) return;
@@ -131,64 +147,105 @@ __global__ __launch_bounds__(kThreads) void k_column_tags(const uint16_t *col, c
 }
 
 // DB scan. Block (query, split) covers rows [split*n/ks, (split+1)*n/ks) of the
-// query's shard and every column; per column it accumulates sum_k brev[k]*h[k] and
-// sum_k key[k]*h[k] as two 32-bit halves in u64 (no overflow for n <= 4096).
+// query's shard and every column. A lane reads four 16-bit column values of a row
+// with one 8-byte load, so a wave streams whole 512-byte rows; per column it
+// accumulates sum_k brev[k]*h[k] and sum_k key[k]*h[k] as two 32-bit halves in u64
+// (no overflow for n <= 4096). brev is b laid out for coefficient 0 of b*h
+// (reverse_for_coeff0, shard_store.rs:378-391).
 __global__ __launch_bounds__(kThreads) void k_scan(CtxDev c, BatchDev bt) {
     const int qi = blockIdx.x, split = blockIdx.y;
     const int n = c.n, gamma = c.gamma;
     const ShardDev sh = c.shards[bt.shard_ids[qi]];
     const uint64_t *b = bt.b + (size_t)qi * n;
     const int rows = n / bt.ks, r0 = split * rows;
-    const int cp = gamma < kThreads ? gamma : kThreads;
-    const int lanes = kThreads / cp;
+    const int G = gamma / 4;                 // 4-column groups per row (gamma >= 4)
+    const int P = kThreads / G;              // rows in flight per block pass
+    const int g = threadIdx.x % G, ro = threadIdx.x / G;
+    uint64_t acc[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) acc[u][0] = acc[u][1] = acc[u][2] = acc[u][3] = 0;
+    const uint64_t *rec = reinterpret_cast<const uint64_t *>(sh.records);
+#pragma unroll 4
+    for (int k = r0 + ro; k < r0 + rows; k += P) {
+        const uint64_t h4 = rec[(size_t)k * G + g];
+        const uint64_t bk = k == 0 ? b[0] : (b[n - k] ? kQ - b[n - k] : 0);
+        const uint64_t key = sh.tag_key[k];
+        const uint32_t bl = (uint32_t)bk, bh = (uint32_t)(bk >> 32);
+        const uint32_t kl = (uint32_t)key, kh = (uint32_t)(key >> 32);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t h = (uint32_t)(h4 >> (16 * u)) & 0xffffu;
+            acc[u][0] += (uint64_t)h * bl;
+            acc[u][1] += (uint64_t)h * bh;
+            acc[u][2] += (uint64_t)h * kl;
+            acc[u][3] += (uint64_t)h * kh;
+        }
+    }
+    __shared__ uint64_t red[kThreads][16];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) red[threadIdx.x][u * 4 + v] = acc[u][v];
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < G * 16; idx += kThreads) {
+        const int gg = idx / 16, slot = idx % 16;
+        uint64_t s = 0;
+        for (int p = 0; p < P; p++) s += red[p * G + gg][slot];
+        const int col = 4 * gg + slot / 4;
+        bt.scan_part[(((size_t)qi * bt.ks + split) * gamma + col) * 4 + (slot % 4)] = s;
+    }
+}
+
+// Narrow shards (gamma < 4 columns): one column per lane.
+__global__ __launch_bounds__(kThreads) void k_scan_narrow(CtxDev c, BatchDev bt) {
+    const int qi = blockIdx.x, split = blockIdx.y;
+    const int n = c.n, gamma = c.gamma;
+    const ShardDev sh = c.shards[bt.shard_ids[qi]];
+    const uint64_t *b = bt.b + (size_t)qi * n;
+    const int rows = n / bt.ks, r0 = split * rows;
+    const int lanes = kThreads / gamma;
+    const int j = threadIdx.x % gamma, lane = threadIdx.x / gamma;
+    uint64_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (int k = r0 + lane; k < r0 + rows; k += lanes) {
+        uint64_t h = sh.records[(size_t)k * gamma + j];
+        uint64_t bk = k == 0 ? b[0] : (b[n - k] ? kQ - b[n - k] : 0);
+        uint64_t key = sh.tag_key[k];
+        a0 += h * (bk & 0xffffffffu);
+        a1 += h * (bk >> 32);
+        a2 += h * (key & 0xffffffffu);
+        a3 += h * (key >> 32);
+    }
     __shared__ uint64_t red[kThreads][4];
-    for (int jb = 0; jb < gamma; jb += cp) {
-        const int j = jb + (int)threadIdx.x % cp, lane = (int)threadIdx.x / cp;
-        uint64_t bl = 0, bh = 0, tl = 0, th = 0;
-        if (lane < lanes && j < gamma) {
-            for (int k = r0 + lane; k < r0 + rows; k += lanes) {
-                uint64_t h = sh.records[(size_t)k * gamma + j];
-                uint64_t bk = k == 0 ? b[0] : (b[n - k] ? kQ - b[n - k] : 0);
-                uint64_t key = sh.tag_key[k];
-                bl += h * (bk & 0xffffffffu);
-                bh += h * (bk >> 32);
-                tl += h * (key & 0xffffffffu);
-                th += h * (key >> 32);
-            }
-        }
-        red[threadIdx.x][0] = bl;
-        red[threadIdx.x][1] = bh;
-        red[threadIdx.x][2] = tl;
-        red[threadIdx.x][3] = th;
-        __syncthreads();
-        if ((int)threadIdx.x < cp && jb + (int)threadIdx.x < gamma) {
-            uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-            for (int l = 0; l < lanes; l++) {
-                int t = l * cp + threadIdx.x;
-                s0 += red[t][0]; s1 += red[t][1]; s2 += red[t][2]; s3 += red[t][3];
-            }
-            uint64_t *o = bt.scan_part + (((size_t)qi * bt.ks + split) * gamma + jb + threadIdx.x) * 4;
-            o[0] = s0; o[1] = s1; o[2] = s2; o[3] = s3;
-        }
-        __syncthreads();
+    red[threadIdx.x][0] = a0; red[threadIdx.x][1] = a1; red[threadIdx.x][2] = a2; red[threadIdx.x][3] = a3;
+    __syncthreads();
+    if ((int)threadIdx.x < gamma) {
+        uint64_t s[4] = {0, 0, 0, 0};
+        for (int l = 0; l < lanes; l++)
+            for (int v = 0; v < 4; v++) s[v] += red[l * gamma + threadIdx.x][v];
+        uint64_t *o = bt.scan_part + (((size_t)qi * bt.ks + split) * gamma + threadIdx.x) * 4;
+        for (int v = 0; v < 4; v++) o[v] = s[v];
     }
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 
+hipError_t launch_scan_only(const CtxDev &c, const BatchDev &b, hipStream_t s) {
+    if (b.count == 0) return hipSuccess;
+    if (c.gamma >= 4)
+        hipLaunchKernelGGL(k_scan, dim3(b.count, b.ks), dim3(kThreads), 0, s, c, b);
+    else
+        hipLaunchKernelGGL(k_scan_narrow, dim3(b.count, b.ks), dim3(kThreads), 0, s, c, b);
+    return hipGetLastError();
+}
+
 hipError_t launch_respond(const CtxDev &c, const BatchDev &b, hipStream_t s) {
     if (b.count == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scan, dim3(b.count, b.ks), dim3(kThreads), 0, s, c, b);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_scan_only(c, b, s);
     if (e != hipSuccess) return e;
     return launch_chain_only(c, b, s);
 }
 
-hipError_t launch_scan_only(const CtxDev &c, const BatchDev &b, hipStream_t s) {
-    hipLaunchKernelGGL(k_scan, dim3(b.count, b.ks), dim3(kThreads), 0, s, c, b);
-    return hipGetLastError();
-}
 
 hipError_t launch_expand_mask(int logn, int rows, const uint8_t *seed, uint64_t stream, int dropped,
                               const uint64_t *psi, uint64_t inv_n, Pair *out, hipStream_t s) {

@@ -16,4 +16,6 @@ hipError_t launch_seat(int logn, int rows, const uint64_t *y, const uint64_t *ps
 hipError_t launch_store_build(const CtxDev &c, const uint16_t *records, uint16_t *col_scratch, const uint64_t *tag_key,
                               Pair *store, uint64_t *tags, hipStream_t s);
 
+hipError_t launch_fold_rows(Pair *rows, int n, uint64_t kx, uint64_t k0, uint64_t k1, hipStream_t s);
+
 }  // namespace inspire

@@ -67,6 +67,7 @@ def lib():
         L.inspire_ctx_destroy.argtypes = [vp]
         L.inspire_ctx_key_rows.argtypes = [vp]
         L.inspire_ctx_key_rows.restype = u32
+        L.inspire_ctx_set_queries_per_cu.argtypes = [vp, u32, cp, sz]
         L.inspire_shard_upload_columns.argtypes = [vp, u32, vp, u32, pp, cp, sz]
         L.inspire_shard_upload_records.argtypes = [vp, u32, vp, sz, sz, pp, cp, sz]
         L.inspire_shard_precompute_packing.argtypes = [vp, vp, cp, sz]
@@ -165,6 +166,9 @@ class Context:
         self.gamma = num_columns
         self.key_rows = lib().inspire_ctx_key_rows(h)
         self.shards = {}
+
+    def set_queries_per_cu(self, k):
+        _call(lib().inspire_ctx_set_queries_per_cu, self._h, k)
 
     def close(self):
         for s in list(self.shards.values()):
