@@ -137,6 +137,31 @@ int inspire_respond_batch_device_timed(inspire_ctx *ctx, const inspire_seat *sea
 /* Copies count statuses (0 = answered, c+1 = column c failed its store tag). */
 int inspire_workspace_status(inspire_workspace *ws, size_t count, int32_t *status, char *err, size_t err_cap);
 
+/* Continuous-batching server over one context (the serving side of
+ * respond_seeded_batch with registered keys, src/pir/respond.rs:1189, and of
+ * ServerSessionStore, src/pir/session.rs:700-896). A worker thread merges queued
+ * requests from any seats into device launches of up to max_launch_queries queries
+ * while the previous launch runs. hbm_budget_bytes caps the seat bytes admitted
+ * (0 = limited by free device memory only). */
+typedef struct inspire_server inspire_server;
+int inspire_server_create(inspire_ctx *ctx, size_t max_launch_queries, size_t hbm_budget_bytes,
+                          inspire_server **out, char *err, size_t err_cap);
+void inspire_server_destroy(inspire_server *server);
+/* ServerSessionStore::register: seat a key in HBM; refused when it does not fit. */
+int inspire_server_register_seat(inspire_server *server, const uint64_t *y_body, uint32_t rows,
+                                 uint32_t num_to_pack, uint64_t *handle_out, char *err, size_t err_cap);
+int inspire_server_unregister_seat(inspire_server *server, uint64_t handle, char *err, size_t err_cap);
+/* Queue one batch (count seeded queries) under a registered seat; returns a ticket. */
+int inspire_server_submit(inspire_server *server, uint64_t handle, const uint32_t *shard_ids,
+                          const uint8_t *seeds, const uint64_t *b, size_t count, uint64_t switch_modulus,
+                          uint64_t *ticket_out, char *err, size_t err_cap);
+/* Block until the ticket's batch is answered; outputs as inspire_respond_batch.
+ * latency_ms receives submit-to-completion time. */
+int inspire_server_wait(inspire_server *server, uint64_t ticket, uint64_t *a_out, uint64_t *b_prefix_out,
+                        double *latency_ms, char *err, size_t err_cap);
+int inspire_server_stats(inspire_server *server, uint64_t *launches, uint64_t *queries,
+                         uint64_t *resident_bytes);
+
 #ifdef __cplusplus
 }
 #endif

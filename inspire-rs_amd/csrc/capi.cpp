@@ -1,187 +1,28 @@
 // Host runtime behind include/inspire_gpu.h: parameter checks, HBM-resident objects,
 // and the launches of the respond pipeline.
-#include <hip/hip_runtime.h>
+#include "runtime.hpp"
 
-#include <cstdarg>
-#include <cstdio>
-#include <cstring>
-#include <fstream>
-#include <mutex>
-#include <new>
-#include <string>
-#include <vector>
-
-#include "../../include/inspire_gpu.h"
-#include "device.hpp"
-#include "kernels.hpp"
-#include "modq.hpp"
-
-using namespace inspire;
-typedef unsigned __int128 u128;
-
-namespace {
-
-void set_err(char *err, size_t cap, const char *fmt, ...) {
-    if (!err || cap == 0) return;
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(err, cap, fmt, ap);
-    va_end(ap);
-}
-
-uint64_t powmod(uint64_t b, uint64_t e, uint64_t m) {
-    uint64_t r = 1;
-    b %= m;
-    while (e) {
-        if (e & 1) r = (uint64_t)((u128)r * b % m);
-        b = (uint64_t)((u128)b * b % m);
-        e >>= 1;
-    }
-    return r;
-}
-uint64_t inv_mod(uint64_t a) { return powmod(a, kQ - 2, kQ); }
-
-// psi: a primitive 2n-th root of unity, g^((q-1)/2n) for the least quadratic
-// non-residue g (NttContext::find_primitive_root, src/math/ntt.rs:743-752).
-uint64_t find_psi(uint64_t two_n) {
-    uint64_t ex = (kQ - 1) / two_n;
-    for (uint64_t g = 2; g < 65536; g++) {
-        uint64_t c = powmod(g, ex, kQ);
-        if (powmod(c, two_n, kQ) == 1 && powmod(c, two_n / 2, kQ) != 1) return c;
-    }
-    return 0;
-}
-
-// ladder[m] = psi^brv(m) built as the reference does (ntt.rs:797-834), standard form.
-std::vector<uint64_t> ladder(size_t n, uint64_t psi) {
-    std::vector<uint64_t> f(n, 0);
-    if (n > 1) f[1] = 1;
-    for (size_t m = 1; m < n; m++) {
-        if ((m & (m - 1)) == 0)
-            f[m] = powmod(psi, n / (2 * m), kQ);
-        else
-            f[m] = (uint64_t)((u128)f[m & (m - 1)] * f[m & (~m + 1)] % kQ);
-    }
-    return f;
-}
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-    hipError_t alloc(size_t b) {
-        if (p) { (void)hipFree(p); p = nullptr; }
-        bytes = b;
-        return hipMalloc(&p, b ? b : 8);
-    }
-    template <class T> T *as() const { return static_cast<T *>(p); }
-};
-
-bool os_random(void *out, size_t len) {
-    std::ifstream f("/dev/urandom", std::ios::binary);
-    if (!f) return false;
-    f.read(static_cast<char *>(out), (std::streamsize)len);
-    return (size_t)f.gcount() == len;
-}
-
-#define HIP_TRY(expr)                                                                  \
-    do {                                                                               \
-        hipError_t e_ = (expr);                                                        \
-        if (e_ != hipSuccess) {                                                        \
-            set_err(err, err_cap, "HIP failure in %s: %s", #expr, hipGetErrorString(e_)); \
-            return INSPIRE_ERR_DEVICE;                                                 \
-        }                                                                              \
-    } while (0)
-
-}  // namespace
-
-struct inspire_shard {
-    inspire_ctx *ctx = nullptr;
-    uint32_t id = 0;
-    DevBuf records, store, tag_key, tags;
-    bool has_store = false;
-};
-
-struct inspire_seat {
-    inspire_ctx *ctx = nullptr;
-    uint32_t rows = 0, num_to_pack = 0;
-    DevBuf yprime;
-};
-
-struct inspire_workspace {
-    inspire_ctx *ctx = nullptr;
-    size_t max_count = 0;
-    DevBuf scan_part, status;
-    // staging for the host-buffer entry point
-    DevBuf ids, seeds, a_in, b, a_out, b_out;
-};
-
-struct inspire_ctx {
-    std::mutex mu;
-    int logn = 0, n = 0, gamma = 0, rows = 0, dropped = 0;
-    uint8_t w_seed[32];
-    DevBuf psi, psi_inv, twf, twi, wprime, gpow, gpow_back, shard_table, col_scratch;
-    uint64_t inv_n = 0, inv_n_gamma = 0;
-    Pair kx{0, 0}, k0{0, 0}, k1{0, 0};
-    std::vector<ShardDev> shards;        // dense by id
-    std::vector<inspire_shard *> owners;
-    bool table_dirty = true;
-    int queries_per_cu = 1;
-    inspire_workspace *default_ws = nullptr;
-
-    int ks() const { return n >= 2048 ? 16 : 1; }
-
-    CtxDev dev() const {
-        CtxDev c;
-        c.logn = logn; c.n = n; c.gamma = gamma; c.rows = rows; c.exact = dropped == 0;
-        c.psi = psi.as<uint64_t>(); c.psi_inv = psi_inv.as<uint64_t>();
-        c.twf = twf.as<Pair>(); c.twi = twi.as<Pair>();
-        c.wprime = wprime.as<Pair>();
-        c.gpow = gpow.as<uint32_t>(); c.gpow_back = gpow_back.as<uint32_t>();
-        c.inv_n_gamma = inv_n_gamma; c.kx = kx; c.k0 = k0; c.k1 = k1;
-        c.shards = shard_table.as<ShardDev>();
-        c.num_shards = (int)shards.size();
-        c.queries_per_cu = queries_per_cu;
-        return c;
-    }
-};
-
-namespace {
-
-int sync_table(inspire_ctx *ctx, char *err, size_t err_cap) {
-    if (!ctx->table_dirty) return INSPIRE_OK;
-    size_t bytes = ctx->shards.size() * sizeof(ShardDev);
-    if (ctx->shard_table.bytes < bytes || !ctx->shard_table.p) HIP_TRY(ctx->shard_table.alloc(bytes * 2 + 64));
-    if (bytes) HIP_TRY(hipMemcpy(ctx->shard_table.p, ctx->shards.data(), bytes, hipMemcpyHostToDevice));
-    ctx->table_dirty = false;
-    return INSPIRE_OK;
-}
-
-int ws_reserve(inspire_workspace *ws, size_t count, char *err, size_t err_cap) {
-    inspire_ctx *ctx = ws->ctx;
-    if (count <= ws->max_count && ws->scan_part.p) return INSPIRE_OK;
-    size_t c = count < 1 ? 1 : count;
-    HIP_TRY(ws->scan_part.alloc(c * ctx->ks() * ctx->gamma * 4 * sizeof(uint64_t)));
-    HIP_TRY(ws->status.alloc(c * sizeof(int32_t)));
-    ws->max_count = c;
-    return INSPIRE_OK;
-}
+namespace inspire_rt {
 
 int build_store(inspire_ctx *ctx, inspire_shard *sh, char *err, size_t err_cap) {
     const size_t n = ctx->n, g = ctx->gamma;
-    std::vector<uint64_t> key(n);
+    // key words below 2^61 - 1, drawn as the reference draws its tag key
+    // (shard_store.rs:127-135): 61 random bits, rejected at or above the prime
+    std::vector<uint64_t> key(n), draw(n);
+    if (!os_random(draw.data(), n * 8)) {
+        set_err(err, err_cap, "OS entropy unavailable for ShardPackingStore::build");
+        return INSPIRE_ERR_DEVICE;
+    }
     for (size_t k = 0; k < n; k++) {
-        // a key word below 2^61 - 1, drawn as the reference draws its tag key
-        // (shard_store.rs:127-135): 61 random bits, rejected at or above the prime
-        for (;;) {
-            uint64_t v;
+        uint64_t v = draw[k] >> 3;
+        while (v >= kTagPrime) {
             if (!os_random(&v, 8)) {
                 set_err(err, err_cap, "OS entropy unavailable for ShardPackingStore::build");
                 return INSPIRE_ERR_DEVICE;
             }
             v >>= 3;
-            if (v < kTagPrime) { key[k] = v; break; }
         }
+        key[k] = v;
     }
     if (!sh->store.p) HIP_TRY(sh->store.alloc(g * n * sizeof(Pair)));
     if (!sh->tag_key.p) HIP_TRY(sh->tag_key.alloc(n * 8));
@@ -201,6 +42,10 @@ int build_store(inspire_ctx *ctx, inspire_shard *sh, char *err, size_t err_cap) 
     ctx->table_dirty = true;
     return sync_table(ctx, err, err_cap);
 }
+
+}  // namespace inspire_rt
+
+namespace {
 
 int register_shard(inspire_ctx *ctx, uint32_t id, const std::vector<uint16_t> &rec, inspire_shard **out, char *err,
                    size_t err_cap) {
@@ -575,6 +420,7 @@ static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_wo
     b.count = (int)count;
     b.shard_ids = d_shard_ids;
     b.seeds = d_seeds;
+    b.yprimes = nullptr;
     b.a_in = nullptr;
     b.b = d_b;
     b.yprime = seat->yprime.as<Pair>();
@@ -690,6 +536,7 @@ static int respond_host(inspire_ctx *ctx, const inspire_seat *seat, const uint32
     bd.shard_ids = ws->ids.as<uint32_t>();
     bd.seeds = ws->seeds.as<uint8_t>();
     bd.a_in = a_in ? ws->a_in.as<uint64_t>() : nullptr;
+    bd.yprimes = nullptr;
     bd.b = ws->b.as<uint64_t>();
     bd.yprime = seat->yprime.as<Pair>();
     bd.scan_part = ws->scan_part.as<uint64_t>();

@@ -218,6 +218,7 @@ __global__ __launch_bounds__((1 << LOGN) / 4, 4) void k_chain(CtxDev c, BatchDev
     const int qi = blockIdx.x, tid = threadIdx.x, gamma = c.gamma;
     const ShardDev sh = c.shards[bt.shard_ids[qi]];
     const Pair *twf = c.twf, *twi = c.twi;
+    const Pair *Y = bt.yprimes ? bt.yprimes[qi] : bt.yprime;
 
     // --- finish the scan: b_poly and the column tags, before any packing work
     if (tid == 0) { bad_column = 0x7fffffff; rejected = 0; }
@@ -344,7 +345,7 @@ __global__ __launch_bounds__((1 << LOGN) / 4, 4) void k_chain(CtxDev c, BatchDev
 #pragma unroll
             for (int rr = 0; rr < ROWS; rr++) {
                 r += shoup(d[rr], c.wprime[(size_t)rr * N + src]);
-                o += shoup(d[rr], bt.yprime[(size_t)rr * N + src]);
+                o += shoup(d[rr], Y[(size_t)rr * N + src]);
             }
             // r < (1 + ROWS) * 2q <= 8q, o < (1 + ROWS) * 2q: back to [0, 2q)
             r = r >= 4 * kQ ? r - 4 * kQ : r;

@@ -43,7 +43,8 @@ struct BatchDev {
     const uint8_t *seeds;       // count x 32 (seeded queries), or
     const uint64_t *a_in;       // count x n expanded a (unseeded queries) when non-null
     const uint64_t *b;          // count x n
-    const Pair *yprime;         // rows x n: n^-1 * NTT(y_body_k) of the seat
+    const Pair *yprime;         // rows x n: n^-1 * NTT(y_body_k) of the seat, or
+    const Pair *const *yprimes; // per-query seats (count pointers) when non-null
     uint64_t *scan_part;        // count x ks x gamma x 4 scratch
     int ks;                     // row splits of the scan
     uint64_t *a_out;            // count x n

@@ -1,0 +1,187 @@
+// Internal runtime objects shared by the C ABI (capi.cpp) and the serving engine
+// (server.cpp). Not part of the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/inspire_gpu.h"
+#include "device.hpp"
+#include "kernels.hpp"
+#include "modq.hpp"
+
+using namespace inspire;
+typedef unsigned __int128 u128;
+
+namespace inspire_rt {
+
+inline void set_err(char *err, size_t cap, const char *fmt, ...) {
+    if (!err || cap == 0) return;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(err, cap, fmt, ap);
+    va_end(ap);
+}
+
+inline uint64_t powmod(uint64_t b, uint64_t e, uint64_t m) {
+    uint64_t r = 1;
+    b %= m;
+    while (e) {
+        if (e & 1) r = (uint64_t)((u128)r * b % m);
+        b = (uint64_t)((u128)b * b % m);
+        e >>= 1;
+    }
+    return r;
+}
+inline uint64_t inv_mod(uint64_t a) { return powmod(a, kQ - 2, kQ); }
+
+// psi: a primitive 2n-th root of unity, g^((q-1)/2n) for the least quadratic
+// non-residue g (NttContext::find_primitive_root, src/math/ntt.rs:743-752).
+inline uint64_t find_psi(uint64_t two_n) {
+    uint64_t ex = (kQ - 1) / two_n;
+    for (uint64_t g = 2; g < 65536; g++) {
+        uint64_t c = powmod(g, ex, kQ);
+        if (powmod(c, two_n, kQ) == 1 && powmod(c, two_n / 2, kQ) != 1) return c;
+    }
+    return 0;
+}
+
+// ladder[m] = psi^brv(m) built as the reference does (ntt.rs:797-834), standard form.
+inline std::vector<uint64_t> ladder(size_t n, uint64_t psi) {
+    std::vector<uint64_t> f(n, 0);
+    if (n > 1) f[1] = 1;
+    for (size_t m = 1; m < n; m++) {
+        if ((m & (m - 1)) == 0)
+            f[m] = powmod(psi, n / (2 * m), kQ);
+        else
+            f[m] = (uint64_t)((u128)f[m & (m - 1)] * f[m & (~m + 1)] % kQ);
+    }
+    return f;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    hipError_t alloc(size_t b) {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        bytes = b;
+        return hipMalloc(&p, b ? b : 8);
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+inline bool os_random(void *out, size_t len) {
+    std::ifstream f("/dev/urandom", std::ios::binary);
+    if (!f) return false;
+    f.read(static_cast<char *>(out), (std::streamsize)len);
+    return (size_t)f.gcount() == len;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            set_err(err, err_cap, "HIP failure in %s: %s", #expr, hipGetErrorString(e_)); \
+            return INSPIRE_ERR_DEVICE;                                                 \
+        }                                                                              \
+    } while (0)
+
+}  // namespace inspire_rt
+
+using namespace inspire_rt;
+
+struct inspire_shard {
+    inspire_ctx *ctx = nullptr;
+    uint32_t id = 0;
+    DevBuf records, store, tag_key, tags;
+    bool has_store = false;
+};
+
+struct inspire_seat {
+    inspire_ctx *ctx = nullptr;
+    uint32_t rows = 0, num_to_pack = 0;
+    DevBuf yprime;
+};
+
+struct inspire_workspace {
+    inspire_ctx *ctx = nullptr;
+    size_t max_count = 0;
+    DevBuf scan_part, status;
+    // staging for the host-buffer entry point
+    DevBuf ids, seeds, a_in, b, a_out, b_out;
+};
+
+struct inspire_ctx {
+    std::mutex mu;
+    int logn = 0, n = 0, gamma = 0, rows = 0, dropped = 0;
+    uint8_t w_seed[32];
+    DevBuf psi, psi_inv, twf, twi, wprime, gpow, gpow_back, shard_table, col_scratch;
+    uint64_t inv_n = 0, inv_n_gamma = 0;
+    Pair kx{0, 0}, k0{0, 0}, k1{0, 0};
+    std::vector<ShardDev> shards;        // dense by id
+    std::vector<inspire_shard *> owners;
+    bool table_dirty = true;
+    int queries_per_cu = 1;
+    inspire_workspace *default_ws = nullptr;
+    std::vector<void *> retired;  // superseded shard tables
+    ~inspire_ctx() {
+        for (void *p : retired) (void)hipFree(p);
+    }
+
+    int ks() const { return n >= 2048 ? 16 : 1; }
+
+    CtxDev dev() const {
+        CtxDev c;
+        c.logn = logn; c.n = n; c.gamma = gamma; c.rows = rows; c.exact = dropped == 0;
+        c.psi = psi.as<uint64_t>(); c.psi_inv = psi_inv.as<uint64_t>();
+        c.twf = twf.as<Pair>(); c.twi = twi.as<Pair>();
+        c.wprime = wprime.as<Pair>();
+        c.gpow = gpow.as<uint32_t>(); c.gpow_back = gpow_back.as<uint32_t>();
+        c.inv_n_gamma = inv_n_gamma; c.kx = kx; c.k0 = k0; c.k1 = k1;
+        c.shards = shard_table.as<ShardDev>();
+        c.num_shards = (int)shards.size();
+        c.queries_per_cu = queries_per_cu;
+        return c;
+    }
+};
+
+namespace inspire_rt {
+
+inline int sync_table(inspire_ctx *ctx, char *err, size_t err_cap) {
+    if (!ctx->table_dirty) return INSPIRE_OK;
+    size_t bytes = ctx->shards.size() * sizeof(ShardDev);
+    if (ctx->shard_table.bytes < bytes || !ctx->shard_table.p) {
+        // a launch in flight may still read the old table: retire it, free it with the context
+        if (ctx->shard_table.p) {
+            ctx->retired.push_back(ctx->shard_table.p);
+            ctx->shard_table.p = nullptr;
+        }
+        HIP_TRY(ctx->shard_table.alloc(bytes * 2 + 64 * sizeof(ShardDev)));
+    }
+    if (bytes) HIP_TRY(hipMemcpy(ctx->shard_table.p, ctx->shards.data(), bytes, hipMemcpyHostToDevice));
+    ctx->table_dirty = false;
+    return INSPIRE_OK;
+}
+
+// Builds a shard's packing store on the device (capi.cpp); caller holds ctx->mu.
+int build_store(inspire_ctx *ctx, inspire_shard *sh, char *err, size_t err_cap);
+
+inline int ws_reserve(inspire_workspace *ws, size_t count, char *err, size_t err_cap) {
+    inspire_ctx *ctx = ws->ctx;
+    if (count <= ws->max_count && ws->scan_part.p) return INSPIRE_OK;
+    size_t c = count < 1 ? 1 : count;
+    HIP_TRY(ws->scan_part.alloc(c * ctx->ks() * ctx->gamma * 4 * sizeof(uint64_t)));
+    HIP_TRY(ws->status.alloc(c * sizeof(int32_t)));
+    ws->max_count = c;
+    return INSPIRE_OK;
+}
+
+}  // namespace inspire_rt

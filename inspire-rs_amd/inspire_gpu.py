@@ -86,6 +86,13 @@ def lib():
         L.inspire_workspace_destroy.argtypes = [vp]
         L.inspire_respond_batch_device.argtypes = [vp, vp, vp, vp, vp, vp, sz, u64, vp, vp, vp, cp, sz]
         L.inspire_respond_batch_device_timed.argtypes = [vp, vp, vp, vp, vp, vp, sz, u64, vp, vp, vp, vp, cp, sz]
+        L.inspire_server_create.argtypes = [vp, sz, sz, pp, cp, sz]
+        L.inspire_server_destroy.argtypes = [vp]
+        L.inspire_server_register_seat.argtypes = [vp, vp, u32, u32, ctypes.POINTER(u64), cp, sz]
+        L.inspire_server_unregister_seat.argtypes = [vp, u64, cp, sz]
+        L.inspire_server_submit.argtypes = [vp, u64, vp, vp, vp, sz, u64, ctypes.POINTER(u64), cp, sz]
+        L.inspire_server_wait.argtypes = [vp, u64, vp, vp, ctypes.POINTER(ctypes.c_double), cp, sz]
+        L.inspire_server_stats.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.inspire_workspace_status.argtypes = [vp, sz, vp, cp, sz]
         _lib = L
     return _lib
@@ -257,6 +264,54 @@ class Context:
         _call(lib().inspire_respond_batch_device, self._h, seat._h, ws._h, ctypes.c_void_p(d_ids),
               ctypes.c_void_p(d_seeds), ctypes.c_void_p(d_b), count, switch_modulus, ctypes.c_void_p(d_a_out),
               ctypes.c_void_p(d_b_out), ctypes.c_void_p(stream))
+
+
+class Server:
+    """Continuous-batching respond server over a Context (inspire_server_*)."""
+
+    def __init__(self, ctx, max_launch=512, hbm_budget=0):
+        h = ctypes.c_void_p()
+        _call(lib().inspire_server_create, ctx._h, max_launch, hbm_budget, ctypes.byref(h))
+        self._h, self.ctx = h, ctx
+
+    def register_seat(self, y_body, num_to_pack=None):
+        """ServerSessionStore::register: returns a handle."""
+        y = np.ascontiguousarray(y_body, dtype=np.uint64)
+        out = ctypes.c_uint64()
+        ntp = self.ctx.gamma if num_to_pack is None else num_to_pack
+        _call(lib().inspire_server_register_seat, self._h, _ptr(y), y.shape[0], ntp, ctypes.byref(out))
+        return out.value
+
+    def unregister_seat(self, handle):
+        _call(lib().inspire_server_unregister_seat, self._h, handle)
+
+    def submit(self, handle, shard_ids, seeds, bs, switch_modulus=0):
+        ids = np.ascontiguousarray(shard_ids, dtype=np.uint32)
+        sd = np.ascontiguousarray(np.frombuffer(b"".join(bytes(s) for s in seeds), dtype=np.uint8))
+        b = np.ascontiguousarray(bs, dtype=np.uint64).reshape(len(ids), self.ctx.n)
+        t = ctypes.c_uint64()
+        _call(lib().inspire_server_submit, self._h, handle, _ptr(ids), _ptr(sd), _ptr(b), len(ids), switch_modulus,
+              ctypes.byref(t))
+        return t.value, len(ids)
+
+    def wait(self, ticket):
+        """(a [count x n], b_prefix [count x gamma], latency_ms) of a submitted batch."""
+        t, count = ticket
+        a = np.zeros((count, self.ctx.n), dtype=np.uint64)
+        b = np.zeros((count, self.ctx.gamma), dtype=np.uint64)
+        lat = ctypes.c_double()
+        _call(lib().inspire_server_wait, self._h, t, _ptr(a), _ptr(b), ctypes.byref(lat))
+        return a, b, lat.value
+
+    def stats(self):
+        l, q, r = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        lib().inspire_server_stats(self._h, ctypes.byref(l), ctypes.byref(q), ctypes.byref(r))
+        return {"launches": l.value, "queries": q.value, "resident_bytes": r.value}
+
+    def close(self):
+        if self._h:
+            lib().inspire_server_destroy(self._h)
+            self._h = None
 
 
 def respond_batch_device_timed(ctx, seat, ws, d_ids, d_seeds, d_b, count, d_a_out, d_b_out, stream=0,

@@ -1,0 +1,98 @@
+"""The continuous-batching server: every batch answered in the mix equals the same batch
+answered alone (and the oracle), requests from many seats merge into shared launches,
+and seats are admitted against the device budget."""
+import threading
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+Q = oracle.Q
+N = 2048
+
+
+def _setup(gpu, approx, gamma, shards, seed):
+    rng = np.random.default_rng(seed)
+    w_seed = rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
+    ctx = gpu.Context(gpu.params_d2048(approx), w_seed, gamma)
+    cols = []
+    for s in range(shards):
+        c = rng.integers(0, 1 << 16, (gamma, N), dtype=np.uint64)
+        cols.append(c)
+        ctx.upload_columns(s, c).precompute_packing()
+    return rng, w_seed, ctx, cols
+
+
+@pytest.mark.parametrize("approx", [True, False])
+def test_mixed_requests_equal_alone(gpu, approx):
+    gamma, shards = 64, 6
+    rng, w_seed, ctx, cols = _setup(gpu, approx, gamma, shards, seed=7 + approx)
+    rows = 2 if approx else 3
+    server = gpu.Server(ctx, max_launch=48)
+    keys = [rng.integers(0, Q, (rows, N), dtype=np.uint64) for _ in range(5)]
+    handles = [server.register_seat(k) for k in keys]
+    reqs = []
+    for r in range(24):
+        count = int(rng.integers(1, 17))
+        kidx = int(rng.integers(0, len(keys)))
+        ids = rng.integers(0, shards, count).astype(np.uint32)
+        seeds = [rng.integers(0, 256, 32, dtype=np.uint8).tobytes() for _ in range(count)]
+        bs = rng.integers(0, Q, (count, N), dtype=np.uint64)
+        reqs.append((kidx, ids, seeds, bs))
+    mod = oracle.Q36 if approx else 0
+    results = [None] * len(reqs)
+
+    def client(part):
+        for i in part:
+            kidx, ids, seeds, bs = reqs[i]
+            t = server.submit(handles[kidx], ids, seeds, bs, switch_modulus=mod)
+            results[i] = server.wait(t)
+
+    threads = [threading.Thread(target=client, args=(list(range(t, len(reqs), 4)),)) for t in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    stats = server.stats()
+    assert stats["queries"] == sum(len(r[1]) for r in reqs)
+    assert stats["launches"] < len(reqs), stats  # requests were merged
+    # alone: each batch through the one-batch entry point under its own seat
+    seats = [ctx.seat(k) for k in keys]
+    for i, (kidx, ids, seeds, bs) in enumerate(reqs):
+        a_alone, b_alone = ctx.respond_seeded_batch(seats[kidx], ids, seeds, bs, switch_modulus=mod)
+        a_mix, b_mix, lat = results[i]
+        assert np.array_equal(a_mix, a_alone) and np.array_equal(b_mix, b_alone), i
+        assert lat > 0
+    # and one query against the oracle
+    kidx, ids, seeds, bs = reqs[0]
+    p = oracle.Pack(N, gamma, 3, 1 if approx else 0)
+    wa, wb = p.respond(w_seed, oracle.expand_seed(seeds[0], N), bs[0], cols[ids[0]], keys[kidx])
+    if mod:
+        wa, wb = oracle.mod_switch(wa), oracle.mod_switch(wb)
+    assert np.array_equal(results[0][0][0], wa) and np.array_equal(results[0][1][0], wb[:gamma])
+    for s in seats:
+        s.close()
+    server.close()
+    ctx.close()
+
+
+def test_admission_and_handles(gpu):
+    gamma = 16
+    rng, w_seed, ctx, cols = _setup(gpu, True, gamma, 1, seed=3)
+    seat_bytes = 2 * N * 16
+    server = gpu.Server(ctx, max_launch=8, hbm_budget=3 * seat_bytes)
+    hs = [server.register_seat(rng.integers(0, Q, (2, N), dtype=np.uint64)) for _ in range(3)]
+    with pytest.raises(gpu.PirError, match="admission refused"):
+        server.register_seat(rng.integers(0, Q, (2, N), dtype=np.uint64))
+    server.unregister_seat(hs[0])
+    h4 = server.register_seat(rng.integers(0, Q, (2, N), dtype=np.uint64))  # fits again
+    assert server.stats()["resident_bytes"] == 3 * seat_bytes
+    with pytest.raises(gpu.PirError, match="no entry for session_handle"):
+        server.submit(hs[0], [0], [bytes(32)], np.zeros((1, N), dtype=np.uint64))
+    a, b, _ = server.wait(server.submit(h4, [0], [bytes(32)], np.zeros((1, N), dtype=np.uint64)))
+    assert a.shape == (1, N) and b.shape == (1, gamma)
+    server.close()
+    ctx.close()
