@@ -292,6 +292,15 @@ def main():
         torch.cuda.synchronize()
     plain_qps = P / (time.perf_counter() - t0)
 
+    # HBM bytes per launch measured by rocprofv3 counters for this workload (512 queries
+    # per launch), committed beside the profile; scaled to this run's launch size
+    traffic_chain = traffic_scan = None
+    tpath = os.path.join(ROOT, "profiles", "respond_batched512_d2048_g256_traffic.json")
+    if os.path.exists(tpath):
+        tj = json.load(open(tpath))
+        traffic_chain = round(tj["k_chain_512_queries_bytes"] * L / 512)
+        traffic_scan = round(tj["k_scan_512_queries_bytes"] * L / 512)
+
     serve = None
     if not args.no_served:
         log("bench: served config")
@@ -335,14 +344,14 @@ def main():
         "vs_plain_gpu": round(qps / plain_qps, 3),
         "roofline": {"bound": "valu", "kernel": "k_chain",
                      "achieved": round(chain_gmms * 4 / 1e3, 3), "peak": VALU_MAD_PEAK_T,
-                     "unit": "T mad32/s", "frac": round(chain_gmms * 4 / 1e3 / VALU_MAD_PEAK_T, 4), "traffic": None,
+                     "unit": "T mad32/s", "frac": round(chain_gmms * 4 / 1e3 / VALU_MAD_PEAK_T, 4), "traffic": traffic_chain,
                      "launch_ms": round(chain_avg, 4), "mm_per_query": mm_per_query,
                      "modmul60_per_s": round(chain_gmms * 1e9, 0),
                      "definition": "4 v_mad_u64_u32 (32x32->64) per 60-bit modular product, algorithmic; peak = "
                                    "256 CU x 64 lanes x 2.4 GHz (v_mad_u64_u32 issues at half the simple-op rate)"},
         "roofline_scan": {"bound": "hbm", "kernel": "k_scan", "achieved": round(scan_gbs, 2),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(scan_gbs / HBM_PEAK_GBS, 4),
-                          "traffic": None, "launch_ms": round(scan_avg, 4)},
+                          "traffic": traffic_scan, "launch_ms": round(scan_avg, 4)},
         "store_stream_gbs": round(chain_gbs, 2),
         "build_s": round(build_s, 2),
         "served": serve,
