@@ -162,6 +162,74 @@ int inspire_server_wait(inspire_server *server, uint64_t ticket, uint64_t *a_out
 int inspire_server_stats(inspire_server *server, uint64_t *launches, uint64_t *queries,
                          uint64_t *resident_bytes);
 
+/* ------------------------------------------------------------------------------
+ * Special-modulus set (src/special_modulus): q < 2^29 and a special prime p < 2^31 on
+ * 32-bit words, 4-bit records, one step-1 table per shard. Replaces
+ * special_modulus::Server::{new, seat, table, respond} (src/special_modulus/server.rs:
+ * 145-279). Coefficient arrays are u32 in [0, m). The wire codecs (wire.rs) and the
+ * client stay on the host.
+ * ---------------------------------------------------------------------------- */
+#define INSPIRE_SM_Q 536813569ULL             /* src/special_modulus/mod.rs:49 */
+#define INSPIRE_SM_P 2147389441ULL            /* src/special_modulus/mod.rs:52 */
+#define INSPIRE_SM_KEY_MASK_VERSION 2         /* src/special_modulus/mod.rs:58 */
+
+typedef struct inspire_sm_server inspire_sm_server;
+typedef struct inspire_sm_table inspire_sm_table;
+typedef struct inspire_sm_seat inspire_sm_seat;
+
+/* special_modulus::Params (src/special_modulus/mod.rs:62-97). */
+typedef struct {
+    uint32_t ring_dim;          /* 2048 served; the gfx950 kernels run 64..2048 */
+    uint32_t gamma;             /* 512 served */
+    uint64_t q;                 /* INSPIRE_SM_Q served; below 2^30 here */
+    uint64_t special_prime;     /* INSPIRE_SM_P served */
+    uint32_t cell_bytes;        /* gamma / 2 */
+    uint32_t record_bytes;      /* 246 served */
+    double sigma;               /* 6.4 served */
+    uint16_t key_mask_version;  /* INSPIRE_SM_KEY_MASK_VERSION */
+} inspire_sm_params;
+
+/* Params::validate (mod.rs:100-165), same messages. */
+int inspire_sm_params_validate(const inspire_sm_params *params, char *err, size_t err_cap);
+/* Server::new / with_kernel (server.rs:145-166): validates, then builds both primes'
+ * transform tables on the device. */
+int inspire_sm_server_create(const inspire_sm_params *params, inspire_sm_server **out, char *err,
+                             size_t err_cap);
+void inspire_sm_server_destroy(inspire_sm_server *server);
+/* Server::table (server.rs:205-250): len bytes of cells, cell_bytes each, row 0 first.
+ * Keeps the cells beside the step-1 table in HBM, so a response is always computed from
+ * the cells its table was built from. */
+int inspire_sm_table_create(inspire_sm_server *server, const uint8_t *cells, size_t len,
+                            inspire_sm_table **out, char *err, size_t err_cap);
+uint32_t inspire_sm_table_rows(const inspire_sm_table *table);          /* ShardTable::rows */
+size_t inspire_sm_table_resident_bytes(const inspire_sm_table *table);  /* device bytes held */
+void inspire_sm_table_destroy(inspire_sm_table *table);
+/* Server::seat (server.rs:177-203): the key's two limbs (coefficient form, ring_dim words
+ * each) and the seed its mask expands from, as PackingKey::from_bytes decoded them. */
+int inspire_sm_seat_create(inspire_sm_server *server, const uint8_t mask_seed[32], const uint32_t *key_q,
+                           const uint32_t *key_p, inspire_sm_seat **out, char *err, size_t err_cap);
+size_t inspire_sm_seat_resident_bytes(const inspire_sm_seat *seat);
+void inspire_sm_seat_destroy(inspire_sm_seat *seat);
+/* Server::respond (server.rs:252-279): one query (32-byte seed, ring_dim body words);
+ * writes the mask (ring_dim words) and the body (2 * record_bytes words). */
+int inspire_sm_respond(inspire_sm_server *server, const inspire_sm_table *table, const inspire_sm_seat *seat,
+                       const uint8_t seed[32], const uint32_t *body, uint32_t *mask_out, uint32_t *body_out,
+                       char *err, size_t err_cap);
+/* count queries in one launch, each with its own table and seat (the loop a caller of
+ * Server::respond runs over a batch); arrays are query-major. */
+int inspire_sm_respond_batch(inspire_sm_server *server, size_t count, const inspire_sm_table *const *tables,
+                             const inspire_sm_seat *const *seats, const uint8_t *seeds, const uint32_t *bodies,
+                             uint32_t *mask_out, uint32_t *body_out, char *err, size_t err_cap);
+/* The same with device-resident seeds, bodies and outputs, launched on stream (a
+ * hipStream_t, NULL for the default stream). Bodies are taken as residues: the host
+ * entry points refuse one that is not. With kernel_ms non-NULL the call waits for the
+ * launch and returns its duration from events on the stream. */
+int inspire_sm_respond_batch_device(inspire_sm_server *server, size_t count,
+                                    const inspire_sm_table *const *tables, const inspire_sm_seat *const *seats,
+                                    const uint8_t *d_seeds, const uint32_t *d_bodies, uint32_t *d_mask_out,
+                                    uint32_t *d_body_out, void *stream, float *kernel_ms, char *err,
+                                    size_t err_cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -94,6 +94,23 @@ def lib():
         L.inspire_server_wait.argtypes = [vp, u64, vp, vp, ctypes.POINTER(ctypes.c_double), cp, sz]
         L.inspire_server_stats.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.inspire_workspace_status.argtypes = [vp, sz, vp, cp, sz]
+        smp = ctypes.POINTER(SmParams)
+        L.inspire_sm_params_validate.argtypes = [smp, cp, sz]
+        L.inspire_sm_server_create.argtypes = [smp, pp, cp, sz]
+        L.inspire_sm_server_destroy.argtypes = [vp]
+        L.inspire_sm_table_create.argtypes = [vp, vp, sz, pp, cp, sz]
+        L.inspire_sm_table_rows.argtypes = [vp]
+        L.inspire_sm_table_rows.restype = u32
+        L.inspire_sm_table_resident_bytes.argtypes = [vp]
+        L.inspire_sm_table_resident_bytes.restype = sz
+        L.inspire_sm_table_destroy.argtypes = [vp]
+        L.inspire_sm_seat_create.argtypes = [vp, vp, vp, vp, pp, cp, sz]
+        L.inspire_sm_seat_resident_bytes.argtypes = [vp]
+        L.inspire_sm_seat_resident_bytes.restype = sz
+        L.inspire_sm_seat_destroy.argtypes = [vp]
+        L.inspire_sm_respond.argtypes = [vp, vp, vp, vp, vp, vp, vp, cp, sz]
+        L.inspire_sm_respond_batch.argtypes = [vp, sz, vp, vp, vp, vp, vp, vp, cp, sz]
+        L.inspire_sm_respond_batch_device.argtypes = [vp, sz, vp, vp, vp, vp, vp, vp, vp, vp, cp, sz]
         _lib = L
     return _lib
 
@@ -351,3 +368,273 @@ def encode_response(a_hat, b_prefix, modulus=Q):
     out += (0).to_bytes(8, "little")  # column_ciphertexts
     out += b"\x01" + (0).to_bytes(4, "little")  # Some(PackingMode::Inspiring)
     return bytes(out)
+
+
+# ---------------------------------------------------------------------------------------
+# Special-modulus set (src/special_modulus): Server / ShardTable / KeySeat on the GPU, and
+# the host-side wire codecs of wire.rs.
+#
+#     SmServer(params)                       Server::new / with_kernel   server.rs:145-166
+#     SmServer.table(cells)                  Server::table               server.rs:205-250
+#     SmServer.seat(key)                     Server::seat                server.rs:177-203
+#     SmServer.respond(table, seat, query)   Server::respond             server.rs:252-279
+#     SmServer.respond_batch(...)            one launch for many (table, seat, query)
+#     PackingKey / Query / Response          wire.rs to_bytes / from_bytes
+# ---------------------------------------------------------------------------------------
+
+SM_Q = 536_813_569
+SM_P = 2_147_389_441
+SM_KEY_MASK_VERSION = 2
+SM_SEED_BYTES = 32
+
+
+class SmParams(ctypes.Structure):
+    """special_modulus::Params (src/special_modulus/mod.rs:62-97)."""
+
+    _fields_ = [
+        ("ring_dim", ctypes.c_uint32),
+        ("gamma", ctypes.c_uint32),
+        ("q", ctypes.c_uint64),
+        ("special_prime", ctypes.c_uint64),
+        ("cell_bytes", ctypes.c_uint32),
+        ("record_bytes", ctypes.c_uint32),
+        ("sigma", ctypes.c_double),
+        ("key_mask_version", ctypes.c_uint16),
+    ]
+
+    def body_coefficients(self):
+        return 2 * self.record_bytes
+
+    def key_bytes(self):
+        return 2 + SM_SEED_BYTES + _packed_bytes(self.ring_dim, self.q) + _packed_bytes(self.ring_dim, self.special_prime)
+
+    def query_bytes(self):
+        return SM_SEED_BYTES + _packed_bytes(self.ring_dim, self.q)
+
+    def response_bytes(self):
+        return _packed_bytes(self.ring_dim, self.q) + _packed_bytes(self.body_coefficients(), self.q)
+
+    def validate(self):
+        _call(lib().inspire_sm_params_validate, ctypes.byref(self))
+
+    def __eq__(self, other):
+        return isinstance(other, SmParams) and all(getattr(self, f) == getattr(other, f) for f, _ in self._fields_)
+
+
+def ring2048_cell256():
+    """Params::ring2048_cell256 (mod.rs:84-97), the served set."""
+    return SmParams(2048, 512, SM_Q, SM_P, 256, 246, 6.4, SM_KEY_MASK_VERSION)
+
+
+def _bits(m):
+    return int(m - 1).bit_length()
+
+
+def _packed_bytes(count, m):
+    return (count * _bits(m) + 7) // 8
+
+
+def pack_tight(values, m):
+    """pack_coefficients_tight at one modulus (src/math/poly.rs:166-224): bit b of value i at
+    bit i*bits + b, least significant first."""
+    v = np.ascontiguousarray(values, dtype=np.uint64)
+    bad = np.nonzero(v >= m)[0]
+    if bad.size:
+        i = int(bad[0])
+        raise PirError(f"tight coefficient {int(v[i])} at index {i} is not canonical for modulus {m}")
+    bits = _bits(m)
+    bitmat = ((v[:, None] >> np.arange(bits, dtype=np.uint64)) & 1).astype(np.uint8).reshape(-1)
+    out = np.packbits(bitmat, bitorder="little")
+    return out.tobytes()[: _packed_bytes(v.size, m)]
+
+
+def unpack_tight(data, m, count):
+    """unpack_coefficients_tight (src/math/poly.rs:227-281): refuses a wrong length, set
+    padding bits and a value at or above the modulus."""
+    bits = _bits(m)
+    want = (count * bits + 7) // 8
+    if len(data) != want:
+        raise PirError(f"tight coefficient payload has {len(data)} bytes, expected {want}")
+    raw = np.frombuffer(bytes(data), dtype=np.uint8)
+    used = (count * bits) % 8
+    if used and int(raw[-1]) & (0xFF ^ ((1 << used) - 1)):
+        raise PirError("tight coefficient payload has nonzero unused high bits")
+    bitv = np.unpackbits(raw, bitorder="little")[: count * bits].reshape(count, bits).astype(np.uint64)
+    vals = (bitv << np.arange(bits, dtype=np.uint64)).sum(axis=1, dtype=np.uint64)
+    bad = np.nonzero(vals >= m)[0]
+    if bad.size:
+        i = int(bad[0])
+        raise PirError(f"tight coefficient {int(vals[i])} at index {i} is not canonical for modulus {m}")
+    return vals.astype(np.uint32)
+
+
+def _split(data, want, what):
+    if len(data) != want:
+        raise PirError(f"{what} has {len(data)} bytes, expected {want}")
+
+
+class PackingKey:
+    """wire::PackingKey (src/special_modulus/wire.rs:33-86)."""
+
+    def __init__(self, version, mask_seed, q, p):
+        self.version, self.mask_seed = int(version), bytes(mask_seed)
+        self.q = np.ascontiguousarray(q, dtype=np.uint32)
+        self.p = np.ascontiguousarray(p, dtype=np.uint32)
+
+    def to_bytes(self, params):
+        return (self.version.to_bytes(2, "big") + self.mask_seed + pack_tight(self.q, params.q)
+                + pack_tight(self.p, params.special_prime))
+
+    @classmethod
+    def from_bytes(cls, params, data):
+        params.validate()
+        d = params.ring_dim
+        _split(data, params.key_bytes(), "packing key")
+        version = int.from_bytes(data[:2], "big")
+        if version != params.key_mask_version:
+            raise PirError(f"packing key is under key-mask version {version}; this set serves version "
+                           f"{params.key_mask_version}")
+        seed = bytes(data[2:2 + SM_SEED_BYTES])
+        rest = data[2 + SM_SEED_BYTES:]
+        at = _packed_bytes(d, params.q)
+        return cls(version, seed, unpack_tight(rest[:at], params.q, d), unpack_tight(rest[at:], params.special_prime, d))
+
+
+class Query:
+    """wire::Query (wire.rs:88-117): the mask seed, then the body at q."""
+
+    def __init__(self, seed, body):
+        self.seed = bytes(seed)
+        self.body = np.ascontiguousarray(body, dtype=np.uint32)
+
+    def to_bytes(self, params):
+        return self.seed + pack_tight(self.body, params.q)
+
+    @classmethod
+    def from_bytes(cls, params, data):
+        _split(data, params.query_bytes(), "query")
+        return cls(data[:SM_SEED_BYTES], unpack_tight(data[SM_SEED_BYTES:], params.q, params.ring_dim))
+
+
+class Response:
+    """wire::Response (wire.rs:119-142): the whole mask, then the body coefficients."""
+
+    def __init__(self, mask, body):
+        self.mask = np.ascontiguousarray(mask, dtype=np.uint32)
+        self.body = np.ascontiguousarray(body, dtype=np.uint32)
+
+    def to_bytes(self, params):
+        return pack_tight(self.mask, params.q) + pack_tight(self.body, params.q)
+
+    @classmethod
+    def from_bytes(cls, params, data):
+        at = _packed_bytes(params.ring_dim, params.q)
+        _split(data, params.response_bytes(), "response")
+        return cls(unpack_tight(data[:at], params.q, params.ring_dim),
+                   unpack_tight(data[at:], params.q, params.body_coefficients()))
+
+    def __eq__(self, other):
+        return np.array_equal(self.mask, other.mask) and np.array_equal(self.body, other.body)
+
+
+class SmTable:
+    """ShardTable (server.rs:97-127): the step-1 table and the cells, both in HBM."""
+
+    def __init__(self, server, handle):
+        self._srv, self._h = server, handle
+
+    def rows(self):
+        return lib().inspire_sm_table_rows(self._h)
+
+    def resident_bytes(self):
+        return lib().inspire_sm_table_resident_bytes(self._h)
+
+    def close(self):
+        if self._h:
+            lib().inspire_sm_table_destroy(self._h)
+            self._h = None
+
+
+class SmSeat:
+    """KeySeat (server.rs:66-92): the key and its mask, both limbs, transform form, in HBM."""
+
+    def __init__(self, server, handle):
+        self._srv, self._h = server, handle
+
+    def bytes(self):
+        return lib().inspire_sm_seat_resident_bytes(self._h)
+
+    def close(self):
+        if self._h:
+            lib().inspire_sm_seat_destroy(self._h)
+            self._h = None
+
+
+class SmServer:
+    """special_modulus::Server on the GPU (server.rs:129-279)."""
+
+    def __init__(self, params):
+        h = ctypes.c_void_p()
+        self.params = params
+        _call(lib().inspire_sm_server_create, ctypes.byref(params), ctypes.byref(h))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().inspire_sm_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def table(self, cells):
+        data = np.frombuffer(bytes(cells), dtype=np.uint8)
+        h = ctypes.c_void_p()
+        _call(lib().inspire_sm_table_create, self._h, _ptr(data), data.size, ctypes.byref(h))
+        return SmTable(self, h)
+
+    def seat(self, key):
+        d = self.params.ring_dim
+        if key.q.size != d or key.p.size != d:
+            raise PirError(f"packing key needs {d} coefficients per limb")
+        seed = np.frombuffer(key.mask_seed, dtype=np.uint8).copy()
+        h = ctypes.c_void_p()
+        _call(lib().inspire_sm_seat_create, self._h, _ptr(seed), _ptr(key.q), _ptr(key.p), ctypes.byref(h))
+        return SmSeat(self, h)
+
+    def respond(self, table, seat, query):
+        masks, bodies = self.respond_batch([table], [seat], [query])
+        return Response(masks[0], bodies[0])
+
+    def respond_batch(self, tables, seats, queries):
+        """count queries, each under its own table and seat, in one launch."""
+        count = len(queries)
+        d, nb = self.params.ring_dim, self.params.body_coefficients()
+        if count == 0:
+            return np.zeros((0, d), np.uint32), np.zeros((0, nb), np.uint32)
+        for qy in queries:
+            if qy.body.size != d:
+                raise PirError("query does not fit the set")
+        th = (ctypes.c_void_p * count)(*[t._h.value for t in tables])
+        sh = (ctypes.c_void_p * count)(*[s._h.value for s in seats])
+        seeds = np.frombuffer(b"".join(qy.seed for qy in queries), dtype=np.uint8).copy()
+        bodies = np.ascontiguousarray(np.stack([qy.body for qy in queries]), dtype=np.uint32)
+        masks = np.zeros((count, d), np.uint32)
+        outs = np.zeros((count, nb), np.uint32)
+        _call(lib().inspire_sm_respond_batch, self._h, count, th, sh, _ptr(seeds), _ptr(bodies), _ptr(masks),
+              _ptr(outs))
+        return masks, outs
+
+    def respond_batch_device(self, tables, seats, d_seeds, d_bodies, d_mask, d_body, stream=0, timed=False):
+        """Device-resident inputs and outputs (addresses); returns the kernel ms when timed."""
+        count = len(tables)
+        th = (ctypes.c_void_p * count)(*[t._h.value for t in tables])
+        sh = (ctypes.c_void_p * count)(*[s._h.value for s in seats])
+        ms = ctypes.c_float(0.0)
+        _call(lib().inspire_sm_respond_batch_device, self._h, count, th, sh, ctypes.c_void_p(d_seeds),
+              ctypes.c_void_p(d_bodies), ctypes.c_void_p(d_mask), ctypes.c_void_p(d_body), ctypes.c_void_p(stream),
+              ctypes.byref(ms) if timed else None)
+        return ms.value if timed else None

@@ -149,6 +149,54 @@ size_t orc_encode_response(size_t n, uint64_t modulus, const uint64_t *a_hat, co
 /* query a from its seed (Poly::from_seed_moduli, poly.rs:682 -> :657) */
 void orc_expand_seed(const uint8_t seed[32], size_t n, uint64_t q, uint64_t *a);
 
+/* ---------------- special modulus (src/special_modulus, special.c) ---------------- */
+typedef struct { uint32_t q, neg_qinv, r2; } sm_prime;
+typedef struct {
+    size_t n;
+    sm_prime p;
+    uint32_t psi;
+    uint32_t *fwd, *inv;  /* psi^(+-bitrev(k)), Montgomery form */
+    uint32_t n_inv, last;
+} sm_ntt;
+typedef struct {
+    size_t d, gamma, g;
+    sm_ntt lq, lp;
+    uint32_t delta, p_mod_q, neg_off, p_over_q, p_inv;
+    uint16_t *tables;     /* gamma x d, or NULL */
+} sm_ring;
+typedef struct sm_client sm_client;
+
+int sm_prime_init(sm_prime *p, uint64_t q);
+uint32_t sm_mont(const sm_prime *p, uint32_t a, uint32_t b);
+uint32_t sm_to_mont(const sm_prime *p, uint32_t a);
+int sm_ntt_init(sm_ntt *c, size_t n, uint64_t q);
+void sm_ntt_free(sm_ntt *c);
+void sm_ntt_forward(const sm_ntt *c, uint32_t *a);
+void sm_ntt_inverse(const sm_ntt *c, uint32_t *a);
+int sm_automorphism_tables(const sm_ntt *c, const size_t *exps, size_t count, uint16_t *out);
+int sm_ring_init(sm_ring *r, size_t d, size_t gamma, uint64_t q, uint64_t p, int with_tables);
+void sm_ring_free(sm_ring *r);
+void sm_expand_query_mask(const sm_ring *r, const uint8_t seed[32], uint32_t *a);
+void sm_key_mask(const sm_ring *r, const uint8_t seed[32], uint32_t *w_q, uint32_t *w_p);
+uint32_t sm_p_to_q_centred(const sm_ring *r, uint32_t x);
+uint32_t sm_q_to_p_centred(const sm_ring *r, uint32_t c);
+uint32_t sm_mod_down(const sm_ring *r, uint32_t xq, uint32_t xp);
+int sm_table(const sm_ring *r, size_t cell_bytes, size_t record_bytes, const uint8_t *cells, size_t len,
+             uint32_t *step1, int8_t *values);
+int sm_seat(const sm_ring *r, const uint32_t *key_q, const uint32_t *key_p, const uint8_t mask_seed[32],
+            uint32_t *sq, uint32_t *sp, uint32_t *w_q, uint32_t *w_p);
+int sm_respond(const sm_ring *r, const uint32_t *step1, const int8_t *values, size_t rows,
+               const uint32_t *sq, const uint32_t *sp, const uint32_t *w_q, const uint32_t *w_p,
+               const uint8_t seed[32], const uint32_t *qbody, uint32_t *mask, uint32_t *body, size_t nb);
+sm_client *sm_client_from_seed(const sm_ring *r, double sigma, const uint8_t seed[32]);
+void sm_client_free(sm_client *c);
+void sm_client_key(const sm_client *c, uint8_t mask_seed[32], uint32_t *key_q, uint32_t *key_p);
+int sm_client_query(sm_client *c, size_t row, uint8_t seed[32], uint32_t *body);
+void sm_client_decode(const sm_client *c, const uint32_t *mask, const uint32_t *body, size_t nb, uint8_t *record);
+size_t sm_packed_bytes(size_t count, uint64_t m);
+size_t sm_pack(const uint32_t *values, size_t count, uint64_t m, uint8_t *out);
+void sm_kat_cells(uint64_t seed, size_t rows, size_t cell_bytes, size_t record_bytes, int worst, uint8_t *out);
+
 #ifdef __cplusplus
 }
 #endif

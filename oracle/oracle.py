@@ -210,3 +210,207 @@ def fnv1a(data: bytes) -> int:
         h ^= x
         h = (h * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
     return h
+
+
+# ---------------- special modulus (special.c) ----------------
+
+SM_Q = 536_813_569        # src/special_modulus/mod.rs:49
+SM_P = 2_147_389_441      # src/special_modulus/mod.rs:52
+SM_D, SM_GAMMA, SM_CELL, SM_RECORD, SM_SIGMA = 2048, 512, 256, 246, 6.4  # mod.rs:84-96
+
+
+class _SmRing(ctypes.Structure):
+    _fields_ = [("opaque", ctypes.c_uint8 * 256)]
+
+
+_i8p = ctypes.POINTER(ctypes.c_int8)
+_lib.sm_ring_init.argtypes = [ctypes.POINTER(_SmRing), ctypes.c_size_t, ctypes.c_size_t, ctypes.c_uint64,
+                              ctypes.c_uint64, ctypes.c_int]
+_lib.sm_ring_free.argtypes = [ctypes.POINTER(_SmRing)]
+_lib.sm_key_mask.argtypes = [ctypes.POINTER(_SmRing), _u8p, _u32p, _u32p]
+_lib.sm_expand_query_mask.argtypes = [ctypes.POINTER(_SmRing), _u8p, _u32p]
+_lib.sm_p_to_q_centred.argtypes = [ctypes.POINTER(_SmRing), ctypes.c_uint32]
+_lib.sm_p_to_q_centred.restype = ctypes.c_uint32
+_lib.sm_q_to_p_centred.argtypes = [ctypes.POINTER(_SmRing), ctypes.c_uint32]
+_lib.sm_q_to_p_centred.restype = ctypes.c_uint32
+_lib.sm_mod_down.argtypes = [ctypes.POINTER(_SmRing), ctypes.c_uint32, ctypes.c_uint32]
+_lib.sm_mod_down.restype = ctypes.c_uint32
+_lib.sm_table.argtypes = [ctypes.POINTER(_SmRing), ctypes.c_size_t, ctypes.c_size_t, _u8p, ctypes.c_size_t,
+                          _u32p, _i8p]
+_lib.sm_seat.argtypes = [ctypes.POINTER(_SmRing), _u32p, _u32p, _u8p, _u32p, _u32p, _u32p, _u32p]
+_lib.sm_respond.argtypes = [ctypes.POINTER(_SmRing), _u32p, _i8p, ctypes.c_size_t, _u32p, _u32p, _u32p, _u32p,
+                            _u8p, _u32p, _u32p, _u32p, ctypes.c_size_t]
+_lib.sm_client_from_seed.argtypes = [ctypes.POINTER(_SmRing), ctypes.c_double, _u8p]
+_lib.sm_client_from_seed.restype = ctypes.c_void_p
+_lib.sm_client_free.argtypes = [ctypes.c_void_p]
+_lib.sm_client_key.argtypes = [ctypes.c_void_p, _u8p, _u32p, _u32p]
+_lib.sm_client_query.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u8p, _u32p]
+_lib.sm_client_decode.argtypes = [ctypes.c_void_p, _u32p, _u32p, ctypes.c_size_t, _u8p]
+_lib.sm_pack.argtypes = [_u32p, ctypes.c_size_t, ctypes.c_uint64, _u8p]
+_lib.sm_pack.restype = ctypes.c_size_t
+_lib.sm_packed_bytes.argtypes = [ctypes.c_size_t, ctypes.c_uint64]
+_lib.sm_packed_bytes.restype = ctypes.c_size_t
+_lib.sm_kat_cells.argtypes = [ctypes.c_uint64, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, _u8p]
+_lib.sm_ntt_forward.argtypes = [ctypes.c_void_p, _u32p]
+_lib.sm_ntt_inverse.argtypes = [ctypes.c_void_p, _u32p]
+
+
+def _p32(a):
+    return a.ctypes.data_as(_u32p)
+
+
+def _u32(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    assert n is None or a.size == n
+    return a
+
+
+def sm_pack(values, m):
+    """The tight codec at the bit width of m (src/special_modulus/wire.rs:10-21)."""
+    v = _u32(values)
+    out = (ctypes.c_uint8 * _lib.sm_packed_bytes(v.size, m))()
+    _lib.sm_pack(_p32(v), v.size, m, ctypes.cast(out, _u8p))
+    return bytes(out)
+
+
+def sm_kat_cells(seed, rows, worst=False, cell_bytes=SM_CELL, record_bytes=SM_RECORD):
+    out = np.zeros(rows * cell_bytes, dtype=np.uint8)
+    _lib.sm_kat_cells(seed, rows, cell_bytes, record_bytes, int(worst), out.ctypes.data_as(_u8p))
+    return out
+
+
+class SmRing:
+    """Ring of one special-modulus parameter set (src/special_modulus/mod.rs:256-363), with the
+    server's table, seat and respond (server.rs)."""
+
+    def __init__(self, d=SM_D, gamma=SM_GAMMA, q=SM_Q, p=SM_P, tables=True, cell_bytes=None, record_bytes=None):
+        self._r = _SmRing()
+        if _lib.sm_ring_init(ctypes.byref(self._r), d, gamma, q, p, int(tables)) != 0:
+            raise ValueError("special-modulus ring init failed")
+        self.d, self.gamma, self.q, self.p = d, gamma, q, p
+        self.cell_bytes = cell_bytes if cell_bytes is not None else gamma // 2
+        self.record_bytes = record_bytes if record_bytes is not None else self.cell_bytes
+        self.nb = 2 * self.record_bytes
+
+    def __del__(self):
+        _lib.sm_ring_free(ctypes.byref(self._r))
+
+    def _lq(self):
+        # sm_ring = {d, gamma, g, lq, lp, ...}: lq sits after three size_t words
+        return ctypes.addressof(self._r) + 3 * 8
+
+    def _lp(self):
+        return self._lq() + 48
+
+    def ntt_q(self, a, inverse=False):
+        a = _u32(a).copy()
+        (_lib.sm_ntt_inverse if inverse else _lib.sm_ntt_forward)(self._lq(), _p32(a))
+        return a
+
+    def ntt_p(self, a, inverse=False):
+        a = _u32(a).copy()
+        (_lib.sm_ntt_inverse if inverse else _lib.sm_ntt_forward)(self._lp(), _p32(a))
+        return a
+
+    def key_mask(self, seed):
+        wq = np.zeros(self.d, np.uint32)
+        wp = np.zeros(self.d, np.uint32)
+        s, _k = _seed(seed)
+        _lib.sm_key_mask(ctypes.byref(self._r), s, _p32(wq), _p32(wp))
+        return wq, wp
+
+    def query_mask(self, seed):
+        a = np.zeros(self.d, np.uint32)
+        s, _k = _seed(seed)
+        _lib.sm_expand_query_mask(ctypes.byref(self._r), s, _p32(a))
+        return a
+
+    def p_to_q_centred(self, x):
+        return _lib.sm_p_to_q_centred(ctypes.byref(self._r), x)
+
+    def q_to_p_centred(self, c):
+        return _lib.sm_q_to_p_centred(ctypes.byref(self._r), c)
+
+    def mod_down(self, xq, xp):
+        return _lib.sm_mod_down(ctypes.byref(self._r), xq, xp)
+
+    def table(self, cells):
+        """(step1 gamma x d, values d x gamma, rows) of Server::table (server.rs:211-250)."""
+        cells = np.ascontiguousarray(np.frombuffer(bytes(cells), np.uint8))
+        step1 = np.zeros((self.gamma, self.d), np.uint32)
+        values = np.zeros((self.d, self.gamma), np.int8)
+        rc = _lib.sm_table(ctypes.byref(self._r), self.cell_bytes, self.record_bytes, cells.ctypes.data_as(_u8p),
+                           cells.size, _p32(step1), values.ctypes.data_as(_i8p))
+        if rc == -1:
+            raise ValueError(f"{cells.size} bytes are not at most {self.d} cells of {self.cell_bytes} bytes")
+        if rc < -1:
+            raise ValueError(f"cell {-2 - rc} has bytes past its {self.record_bytes}-byte record")
+        return step1, values, cells.size // self.cell_bytes
+
+    def seat(self, key_q, key_p, mask_seed):
+        outs = [np.zeros(self.d, np.uint32) for _ in range(4)]
+        s, _k = _seed(mask_seed)
+        kq, kp = _u32(key_q, self.d), _u32(key_p, self.d)
+        if _lib.sm_seat(ctypes.byref(self._r), _p32(kq), _p32(kp), s, *[_p32(o) for o in outs]) != 0:
+            raise ValueError("packing key coefficient is not a residue")
+        return tuple(outs)  # (key_q, key_p, w_q, w_p), transform form, Montgomery operands
+
+    def respond(self, table, seat, seed, body):
+        step1, values, rows = table
+        mask = np.zeros(self.d, np.uint32)
+        out = np.zeros(self.nb, np.uint32)
+        s, _k = _seed(seed)
+        b = _u32(body, self.d)
+        if _lib.sm_respond(ctypes.byref(self._r), _p32(step1), values.ctypes.data_as(_i8p), rows,
+                           *[_p32(x) for x in seat], s, _p32(b), _p32(mask), _p32(out), self.nb) != 0:
+            raise ValueError("query does not fit the set")
+        return mask, out
+
+    def response_bytes(self, mask, body):
+        return sm_pack(mask, self.q) + sm_pack(body, self.q)
+
+
+class SmClient:
+    """Client::from_seed (src/special_modulus/client.rs:137-244) and its queries."""
+
+    def __init__(self, ring, seed, sigma=SM_SIGMA):
+        self.ring = ring
+        s, _k = _seed(seed)
+        self._c = _lib.sm_client_from_seed(ctypes.byref(ring._r), sigma, s)
+
+    def __del__(self):
+        _lib.sm_client_free(self._c)
+
+    def packing_key(self):
+        d = self.ring.d
+        seed = (ctypes.c_uint8 * 32)()
+        kq, kp = np.zeros(d, np.uint32), np.zeros(d, np.uint32)
+        _lib.sm_client_key(self._c, ctypes.cast(seed, _u8p), _p32(kq), _p32(kp))
+        return bytes(seed), kq, kp
+
+    def key_bytes(self, version=2):
+        seed, kq, kp = self.packing_key()
+        return version.to_bytes(2, "big") + seed + sm_pack(kq, self.ring.q) + sm_pack(kp, self.ring.p)
+
+    def query(self, row):
+        seed = (ctypes.c_uint8 * 32)()
+        body = np.zeros(self.ring.d, np.uint32)
+        if _lib.sm_client_query(self._c, row, ctypes.cast(seed, _u8p), _p32(body)) != 0:
+            raise ValueError(f"row {row} is past the {self.ring.d} rows of a shard")
+        return bytes(seed), body
+
+    def decode(self, mask, body):
+        body = _u32(body)
+        out = (ctypes.c_uint8 * (body.size // 2))()
+        _lib.sm_client_decode(self._c, _p32(_u32(mask)), _p32(body), body.size, ctypes.cast(out, _u8p))
+        return bytes(out)
+
+
+def _sm_tables(ring):
+    """The ring's automorphism tables (gamma x d u16), as the reference searched them."""
+    addr = ctypes.addressof(ring._r) + 3 * 8 + 2 * 48 + 5 * 4 + 4
+    ptr = ctypes.c_void_p.from_address(addr).value
+    return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_uint16)), shape=(ring.gamma, ring.d)).copy()
+
+
+SmRing.tables = _sm_tables

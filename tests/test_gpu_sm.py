@@ -1,0 +1,164 @@
+"""The special-modulus path on the GPU, through the C ABI, against the oracle
+(oracle/special.c, itself pinned to the reference's served-set wire digests)."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+SM_Q, SM_P = oracle.SM_Q, oracle.SM_P
+PINNED = [0x84FE10759FE96D5F, 0xD889F6A134B7E313, 0xEF10B409C5474E4B]
+
+
+def _small(gpu, d, gamma):
+    # tests/special_modulus_kat.rs:26-34: cells of gamma/2 bytes, all of them record
+    return gpu.SmParams(d, gamma, SM_Q, SM_P, gamma // 2, gamma // 2, 6.4, 2)
+
+
+def _oracle_ring(params):
+    return oracle.SmRing(d=params.ring_dim, gamma=params.gamma, cell_bytes=params.cell_bytes,
+                         record_bytes=params.record_bytes)
+
+
+def _random_key(gpu, rng, d):
+    return gpu.PackingKey(2, bytes(rng.integers(0, 256, 32, dtype=np.uint8)),
+                          rng.integers(0, SM_Q, d, dtype=np.uint32), rng.integers(0, SM_P, d, dtype=np.uint32))
+
+
+def _random_query(gpu, rng, d):
+    return gpu.Query(bytes(rng.integers(0, 256, 32, dtype=np.uint8)), rng.integers(0, SM_Q, d, dtype=np.uint32))
+
+
+def test_served_wire_bytes_are_pinned(gpu):
+    params = gpu.ring2048_cell256()
+    ring = oracle.SmRing(cell_bytes=256, record_bytes=246)
+    client = oracle.SmClient(ring, bytes([42] * 32))
+    cells = oracle.sm_kat_cells(42, 2048)
+    with gpu.SmServer(params) as server:
+        table = server.table(cells.tobytes())
+        assert table.rows() == 2048
+        key_bytes = client.key_bytes()
+        seat = server.seat(gpu.PackingKey.from_bytes(params, key_bytes))
+        qseed, qbody = client.query(1234)
+        query_bytes = gpu.Query(qseed, qbody).to_bytes(params)
+        response = server.respond(table, seat, gpu.Query.from_bytes(params, query_bytes))
+        response_bytes = response.to_bytes(params)
+        assert (len(key_bytes), len(query_bytes), len(response_bytes)) == (15_394, 7_456, 9_208)
+        assert [oracle.fnv1a(key_bytes), oracle.fnv1a(query_bytes), oracle.fnv1a(response_bytes)] == PINNED
+        assert client.decode(response.mask, response.body) == bytes(cells[1234 * 256:1234 * 256 + 246])
+        seat.close()
+        table.close()
+
+
+@pytest.mark.parametrize("d,gamma", [(256, 32), (512, 64), (256, 128), (64, 8), (1024, 64), (128, 16)])
+def test_small_rings_match_the_oracle(gpu, d, gamma):
+    params = _small(gpu, d, gamma)
+    ring = _oracle_ring(params)
+    rng = np.random.default_rng(1000 + d + gamma)
+    worst = np.full(d * params.cell_bytes, 0x88, dtype=np.uint8)  # every nibble -8: the largest fold
+    rand_rows = int(rng.integers(1, d + 1))
+    cells = [worst.tobytes(), rng.integers(0, 256, rand_rows * params.cell_bytes, dtype=np.uint8).tobytes()]
+    with gpu.SmServer(params) as server:
+        tables = [server.table(c) for c in cells]
+        keys = [_random_key(gpu, rng, d) for _ in range(2)]
+        seats = [server.seat(k) for k in keys]
+        queries = [_random_query(gpu, rng, d) for _ in range(4)]
+        pick_t, pick_s = [0, 1, 1, 0], [0, 0, 1, 1]
+        masks, bodies = server.respond_batch([tables[i] for i in pick_t], [seats[i] for i in pick_s], queries)
+        otables = [ring.table(c) for c in cells]
+        oseats = [ring.seat(k.q, k.p, k.mask_seed) for k in keys]
+        for j, qy in enumerate(queries):
+            wm, wb = ring.respond(otables[pick_t[j]], oseats[pick_s[j]], qy.seed, qy.body)
+            assert np.array_equal(masks[j], wm), (d, gamma, j)
+            assert np.array_equal(bodies[j], wb), (d, gamma, j)
+        for x in tables + seats:
+            x.close()
+
+
+def test_served_batch_mixed_tables_and_seats_equal_alone_and_oracle(gpu):
+    params = gpu.ring2048_cell256()
+    ring = oracle.SmRing(cell_bytes=256, record_bytes=246)
+    rng = np.random.default_rng(77)
+    cells = [oracle.sm_kat_cells(77, 2048).tobytes(), oracle.sm_kat_cells(78, 1500).tobytes(),
+             oracle.sm_kat_cells(79, 2048, worst=True).tobytes(), b""]
+    with gpu.SmServer(params) as server:
+        tables = [server.table(c) for c in cells]
+        assert [t.rows() for t in tables] == [2048, 1500, 2048, 0]
+        keys = [_random_key(gpu, rng, 2048) for _ in range(2)]
+        seats = [server.seat(k) for k in keys]
+        assert seats[0].bytes() == 32_768
+        pick_t, pick_s = [0, 1, 2, 3, 2, 0], [0, 1, 0, 1, 1, 1]
+        queries = [_random_query(gpu, rng, 2048) for _ in pick_t]
+        masks, bodies = server.respond_batch([tables[i] for i in pick_t], [seats[i] for i in pick_s], queries)
+        otables = [ring.table(c) for c in cells[:3]]
+        oseats = [ring.seat(k.q, k.p, k.mask_seed) for k in keys]
+        for j, qy in enumerate(queries):
+            alone = server.respond(tables[pick_t[j]], seats[pick_s[j]], qy)
+            assert np.array_equal(alone.mask, masks[j]) and np.array_equal(alone.body, bodies[j]), j
+            if pick_t[j] < 3:
+                wm, wb = ring.respond(otables[pick_t[j]], oseats[pick_s[j]], qy.seed, qy.body)
+                assert np.array_equal(masks[j], wm) and np.array_equal(bodies[j], wb), j
+        for x in tables + seats:
+            x.close()
+
+
+def test_served_records_decode_at_batch_scale(gpu):
+    # tests/special_modulus_kat.rs:296-336 at a full batch: every response of a 96-query
+    # launch over four tables and four client keys decodes to its record
+    params = gpu.ring2048_cell256()
+    ring = oracle.SmRing(cell_bytes=256, record_bytes=246)
+    rng = np.random.default_rng(5)
+    rows = [2048, 2048, 1500, 700]
+    cells = [oracle.sm_kat_cells(200 + i, r) for i, r in enumerate(rows)]
+    clients = [oracle.SmClient(ring, bytes([i + 1] * 32)) for i in range(4)]
+    with gpu.SmServer(params) as server:
+        tables = [server.table(c.tobytes()) for c in cells]
+        seats = [server.seat(gpu.PackingKey.from_bytes(params, c.key_bytes())) for c in clients]
+        picks, queries = [], []
+        for j in range(96):
+            ti, ci = int(rng.integers(0, 4)), int(rng.integers(0, 4))
+            row = int(rng.integers(0, 2048))
+            seed, body = clients[ci].query(row)
+            picks.append((ti, ci, row))
+            queries.append(gpu.Query(seed, body))
+        masks, bodies = server.respond_batch([tables[t] for t, _, _ in picks], [seats[c] for _, c, _ in picks],
+                                             queries)
+        for j, (ti, ci, row) in enumerate(picks):
+            want = bytes(cells[ti][row * 256:row * 256 + 246]) if row < rows[ti] else bytes(246)
+            assert clients[ci].decode(masks[j], bodies[j]) == want, (j, ti, ci, row)
+        for x in tables + seats:
+            x.close()
+
+
+def test_refusals(gpu):
+    params = gpu.ring2048_cell256()
+    rng = np.random.default_rng(1)
+    with gpu.SmServer(params) as server, gpu.SmServer(params) as other:
+        cells = bytearray(oracle.sm_kat_cells(3, 4).tobytes())
+        cells[3 * 256 + 250] = 1
+        with pytest.raises(gpu.PirError, match="cell 3 has bytes past its 246-byte record"):
+            server.table(bytes(cells))
+        with pytest.raises(gpu.PirError, match="not at most 2048 cells of 256 bytes"):
+            server.table(bytes(300))
+        with pytest.raises(gpu.PirError, match="not at most 2048 cells"):
+            server.table(bytes(2049 * 256))
+        key = _random_key(gpu, rng, 2048)
+        bad = gpu.PackingKey(2, key.mask_seed, key.q.copy(), key.p.copy())
+        bad.p[7] = SM_P
+        with pytest.raises(gpu.PirError, match="not a residue"):
+            server.seat(bad)
+        table, seat = server.table(b""), server.seat(key)
+        q = _random_query(gpu, rng, 2048)
+        q.body[5] = SM_Q
+        with pytest.raises(gpu.PirError, match="query does not fit the set"):
+            server.respond(table, seat, q)
+        foreign = other.table(b"")
+        with pytest.raises(gpu.PirError, match="another parameter set"):
+            server.respond(foreign, seat, _random_query(gpu, rng, 2048))
+        for x in (table, seat, foreign):
+            x.close()
+    bad_params = gpu.ring2048_cell256()
+    bad_params.key_mask_version = 1
+    with pytest.raises(gpu.PirError, match="per-key-seed version"):
+        gpu.SmServer(bad_params)
