@@ -152,6 +152,121 @@ def served(g, seconds, rate_qps, seed=7):
                         "per launch, 36-bit switch"}
 
 
+SM_N, SM_GAMMA, SM_CELL, SM_RECORD = 2048, 512, 256, 246
+# 32-bit modular products per special-modulus query: 511 steps x (4 transforms of 11264
+# butterflies + 6 x 2048 pointwise), plus NTT(a), the first step and the final 3 transforms
+SM_MM_PER_QUERY = 511 * (4 * 11264 + 6 * 2048) + 11264 + 2048 + 3 * 11264 + 492
+# v_mul_lo_u32 / v_mul_hi_u32 issue at 64 lanes per clock per CU (half the simple-op rate)
+MUL32_PEAK_T = 256 * 64 * 2.4e9 / 1e12
+
+
+def sm_cells(rng, rows):
+    c = np.frombuffer(rng.bytes(rows * SM_CELL), dtype=np.uint8).reshape(rows, SM_CELL).copy()
+    c[:, SM_RECORD:] = 0
+    return c.tobytes()
+
+
+def sm_cpu_baseline(budget_s):
+    """The CPU oracle's special-modulus respond (C restatement of respond_impl, one thread) on a
+    bounded sample of served-set queries against one prebuilt table and seat."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    rng = np.random.default_rng(98)
+    ring = oracle.SmRing(cell_bytes=SM_CELL, record_bytes=SM_RECORD)
+    table = ring.table(sm_cells(rng, SM_N))
+    seat = ring.seat(rng.integers(0, oracle.SM_Q, SM_N, dtype=np.uint32),
+                     rng.integers(0, oracle.SM_P, SM_N, dtype=np.uint32), bytes(32))
+    done, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s or done == 0:
+        ring.respond(table, seat, rng.bytes(32), rng.integers(0, oracle.SM_Q, SM_N, dtype=np.uint32))
+        done += 1
+    dt = time.perf_counter() - t0
+    return {"value": round(done / dt, 3), "unit": "queries/s", "cores": 1, "kind": "port",
+            "sample": f"{done} ring2048_cell256 queries against one prebuilt table (C restatement, 1 thread), "
+                      f"{dt:.1f} s"}
+
+
+def special_modulus(g, torch, dev, stream, args):
+    """Served special-modulus set (ring2048_cell256): 2^22 cells of 256 B in 2048 step-1 tables
+    (one per shard of 2048 rows), 256 seated keys with their own mask seeds, batches of queries
+    at uniform shards under uniform keys, one launch per batch, inputs resident in HBM."""
+    rng = np.random.default_rng(4242)
+    params = g.ring2048_cell256()
+    srv = g.SmServer(params)
+    t0 = time.perf_counter()
+    tables = [srv.table(sm_cells(rng, SM_N)) for _ in range(args.sm_tables)]
+    torch.cuda.synchronize()
+    table_s = time.perf_counter() - t0
+    seats = [srv.seat(g.PackingKey(2, rng.bytes(32), rng.integers(0, g.SM_Q, SM_N, dtype=np.uint32),
+                                   rng.integers(0, g.SM_P, SM_N, dtype=np.uint32))) for _ in range(args.sm_keys)]
+    L, K = args.sm_batch, args.sm_steps
+    total = 1 + K
+    d_seeds = torch.from_numpy(rng.integers(0, 256, (total, L, 32), dtype=np.uint8)).to(dev)
+    d_bodies = torch.from_numpy(rng.integers(0, g.SM_Q, (total, L, SM_N), dtype=np.int64).astype(np.int32)).to(dev)
+    d_mask = torch.empty((L, SM_N), dtype=torch.int32, device=dev)
+    d_body = torch.empty((L, 2 * SM_RECORD), dtype=torch.int32, device=dev)
+    picks = [([tables[i] for i in rng.integers(0, len(tables), L)], [seats[i] for i in rng.integers(0, len(seats), L)])
+             for _ in range(total)]
+
+    def issue(i, timed=False):
+        return srv.respond_batch_device(picks[i][0], picks[i][1], d_seeds[i].data_ptr(), d_bodies[i].data_ptr(),
+                                        d_mask.data_ptr(), d_body.data_ptr(), stream=stream, timed=timed)
+
+    issue(0)
+    torch.cuda.synchronize()
+    # the job list of each launch is uploaded by the call when its handles change; measure the
+    # launches back to back on one stream with their own job lists
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * K)]
+    t0 = time.perf_counter()
+    for k in range(K):
+        ev[2 * k].record()
+        issue(1 + k)
+        ev[2 * k + 1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([ev[2 * k].elapsed_time(ev[2 * k + 1]) for k in range(K)])
+    kern = [issue(1 + k, timed=True) for k in range(min(K, 3))]
+    kms = float(np.mean(kern))
+    qps = L * K / wall
+    mm_s = L * SM_MM_PER_QUERY / (kms * 1e-3)
+    # per query the kernel streams its table (gamma x n words) and reads its cells, seat and body
+    bytes_per_query = SM_GAMMA * SM_N * 4 + SM_N * SM_CELL + 4 * SM_N * 4 + SM_N * 4 + SM_N * 4 + 2 * SM_RECORD * 4
+    # HBM bytes per launch measured by rocprofv3 counters for this workload (2048 queries per
+    # launch), committed beside the profile; scaled to this run's launch size
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "sm_respond_batched2048_ring2048_cell256_traffic.json")
+    if os.path.exists(tpath):
+        tj = json.load(open(tpath))
+        traffic = round(tj["hbm_bytes_per_dispatch"]["inspire::k_sm_respond<11>/2048"] * L / 2048)
+    out = {
+        "value": round(qps, 1), "unit": "queries/s",
+        "workload": f"special_modulus ring2048_cell256: {len(tables)} step-1 tables ({len(tables) * SM_N} cells of "
+                    f"{SM_CELL} B), {len(seats)} seated keys, {L} queries per launch at uniform tables and keys, "
+                    f"{K} launches back to back, inputs resident in HBM",
+        "queries_per_launch": L,
+        "launch_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
+        "kernel_ms": round(kms, 3),
+        "table_build_s": round(table_s, 2),
+        "tables_resident_bytes": sum(t.resident_bytes() for t in tables),
+        "seat_bytes": seats[0].bytes(),
+        "roofline": {"bound": "valu", "kernel": "k_sm_respond",
+                     "achieved": round(mm_s * 3 / 1e12, 3), "peak": MUL32_PEAK_T, "unit": "T mul32/s",
+                     "frac": round(mm_s * 3 / 1e12 / MUL32_PEAK_T, 4), "traffic": traffic,
+                     "algorithmic_bytes": L * bytes_per_query,
+                     "launch_ms": round(kms, 4), "mm32_per_query": SM_MM_PER_QUERY,
+                     "hbm_gbs": round(L * bytes_per_query / (kms * 1e-3) / 1e9, 1),
+                     "definition": "3 v_mul_lo/hi_u32 per 32-bit modular product (Shoup or Montgomery), "
+                                   "algorithmic; peak = 256 CU x 64 lanes x 2.4 GHz"},
+    }
+    for t in tables:
+        t.close()
+    for st in seats:
+        st.close()
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +282,12 @@ def main():
     ap.add_argument("--plain-queries", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sm-tables", type=int, default=2048, help="special-modulus step-1 tables (2048 = 2^22 cells)")
+    ap.add_argument("--sm-keys", type=int, default=256)
+    ap.add_argument("--sm-batch", type=int, default=2048, help="special-modulus queries per launch")
+    ap.add_argument("--sm-steps", type=int, default=4)
+    ap.add_argument("--sm-only", action="store_true", help="run only the special-modulus workload")
+    ap.add_argument("--no-sm", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy a 256 MiB device buffer once: a kernel of known HBM traffic that "
                          "scales FETCH_SIZE / WRITE_SIZE counters read from the same profiled run")
@@ -192,6 +313,14 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+
+    if args.sm_only:
+        log("bench: special-modulus workload only")
+        sm = special_modulus(g, torch, dev, stream, args)
+        if not args.no_cpu:
+            sm["cpu_baseline"] = sm_cpu_baseline(args.cpu_seconds)
+        print(json.dumps({"special_modulus": sm}), flush=True)
+        return
 
     log(f"bench: building {args.shards} shards of {N} x {ENTRY} B with stores")
     ctx = g.Context(g.params_d2048(), w_seed, GAMMA)
@@ -301,6 +430,11 @@ def main():
         traffic_chain = round(tj["k_chain_512_queries_bytes"] * L / 512)
         traffic_scan = round(tj["k_scan_512_queries_bytes"] * L / 512)
 
+    sm = None
+    if not args.no_sm:
+        log("bench: special-modulus workload")
+        sm = special_modulus(g, torch, dev, stream, args)
+
     serve = None
     if not args.no_served:
         log("bench: served config")
@@ -311,6 +445,8 @@ def main():
         log("bench: CPU baseline")
         cpu = cpu_baseline(args.cpu_seconds)
         cpu["cores"] = 1
+        if sm is not None:
+            sm["cpu_baseline"] = sm_cpu_baseline(min(args.cpu_seconds, 10.0))
 
     result = {
         "metric": "respond queries/s + DB GB/s (byte-exact) vs plain GPU; p50/p99 latency",
@@ -355,6 +491,7 @@ def main():
         "store_stream_gbs": round(chain_gbs, 2),
         "build_s": round(build_s, 2),
         "served": serve,
+        "special_modulus": sm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)

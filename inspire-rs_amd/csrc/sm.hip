@@ -198,6 +198,27 @@ __device__ __forceinline__ void fwd_pass(uint32_t v[8], int t, const Pair32 *__r
     }
 }
 
+// Ordering between two passes whose groups span slot bits below HI. Each 64-lane wave
+// owns the 512 slots 512w .. 512w+511 in every pass that stays below bit 9 (8 slots per
+// lane: the own pass, and the passes over bits 3-5 and 6-8 whose groups are 8 and 64
+// consecutive lanes), so those exchanges only need the wave's own LDS order; a pass
+// reaching bit 9 or above mixes the waves and takes the workgroup barrier.
+template <int HI>
+__device__ __forceinline__ void pass_sync() {
+    if constexpr (HI <= 9) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+// top bit of the pass that follows the pass over bits [B, B+3) going up
+template <int LOGN, int B>
+struct NextUp {
+    static constexpr int value = (B + 3 < Plan<LOGN>::BT) ? B + 6 : LOGN;
+};
+
 // radix-8 passes from bit B up to (not including) the top pass, through LDS
 template <int LOGN, int B, bool P>
 __device__ __forceinline__ void inv_mid(uint32_t *X, const Pair32 *tw, uint32_t m, int t) {
@@ -206,7 +227,7 @@ __device__ __forceinline__ void inv_mid(uint32_t *X, const Pair32 *tw, uint32_t 
         lds_load<LOGN, B, 3>(X, v, t);
         inv_pass<LOGN, B, 3, P>(v, t, tw, m);
         lds_store<LOGN, B, 3>(X, v, t);
-        __syncthreads();
+        pass_sync<NextUp<LOGN, B>::value>();
         inv_mid<LOGN, B + 3, P>(X, tw, m, t);
     }
 }
@@ -218,7 +239,7 @@ __device__ __forceinline__ void fwd_mid(uint32_t *X, const Pair32 *tw, uint32_t 
         lds_load<LOGN, B, 3>(X, v, t);
         fwd_pass<LOGN, B, 3, P>(v, t, tw, m);
         lds_store<LOGN, B, 3>(X, v, t);
-        __syncthreads();
+        pass_sync<B + 3>();
         fwd_mid<LOGN, B - 3, P>(X, tw, m, t);
     }
 }
@@ -242,7 +263,7 @@ __device__ __forceinline__ void round_trip(uint32_t v[8], uint32_t *X, const SmC
     const uint32_t mi = QP ? c.q : c.p, mf = QP ? c.p : c.q;
     inv_pass<LOGN, 0, 3, !QP>(v, t, twi, mi);
     own_store(X, v, t);
-    __syncthreads();
+    pass_sync<NextUp<LOGN, 0>::value>();
     inv_mid<LOGN, 3, !QP>(X, twi, mi, t);
     {
         uint32_t u[8];
@@ -253,7 +274,7 @@ __device__ __forceinline__ void round_trip(uint32_t v[8], uint32_t *X, const SmC
         fwd_pass<LOGN, BT, RT, QP>(u, t, twf, mf);
         lds_store<LOGN, BT, RT>(X, u, t);
     }
-    __syncthreads();
+    pass_sync<LOGN>();
     fwd_mid<LOGN, BT - 3, QP>(X, twf, mf, t);
     own_load(X, v, t);
     fwd_pass<LOGN, 0, 3, QP>(v, t, twf, mf);
@@ -269,7 +290,7 @@ __device__ __forceinline__ void forward_from_lds(uint32_t *X, uint32_t v[8], con
         fwd_pass<LOGN, BT, RT, P>(u, t, tw, m);
         lds_store<LOGN, BT, RT>(X, u, t);
     }
-    __syncthreads();
+    pass_sync<LOGN>();
     fwd_mid<LOGN, BT - 3, P>(X, tw, m, t);
     own_load(X, v, t);
     fwd_pass<LOGN, 0, 3, P>(v, t, tw, m);
@@ -282,7 +303,7 @@ __device__ __forceinline__ void inverse_to_lds(uint32_t v[8], uint32_t *X, const
     constexpr int BT = Plan<LOGN>::BT, RT = Plan<LOGN>::RT;
     inv_pass<LOGN, 0, 3, P>(v, t, tw, m);
     own_store(X, v, t);
-    __syncthreads();
+    pass_sync<NextUp<LOGN, 0>::value>();
     inv_mid<LOGN, 3, P>(X, tw, m, t);
     {
         uint32_t u[8];

@@ -26,6 +26,10 @@
  *   approximate gadget -> decode correctness only: "parity unpinned" at the
  *       byte level (no committed reference vector exists).
  *   mod switch -> u128 closed form brute force.
+ *   special modulus (special.c), the whole chain: the served-set wire digests of
+ *       tests/special_modulus_kat.rs:349-384 (client key, query and response bytes);
+ *       Ntt32 by O(n^2) products; automorphism tables by the invariant and the
+ *       closed form; lifts and mod_down by i128 brute force; decode of every record.
  */
 #ifndef INSPIRE_ORACLE_H
 #define INSPIRE_ORACLE_H
