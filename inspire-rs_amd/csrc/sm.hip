@@ -256,9 +256,9 @@ __device__ __forceinline__ uint32_t lift_pq(uint32_t r, const SmCtxDev &c) {
 
 // iNTT of the own-slot registers v, centred lift, NTT: QP maps q -> p, else p -> q.
 // Leaves v in the other prime's transform domain (mod p canonical, mod q in [0, 4q)).
-template <int LOGN, bool QP>
+template <int LOGN, bool QP, class Mid>
 __device__ __forceinline__ void round_trip(uint32_t v[8], uint32_t *X, const SmCtxDev &c, const Pair32 *twi,
-                                           const Pair32 *twf, int t) {
+                                           const Pair32 *twf, int t, Mid &&mid) {
     constexpr int BT = Plan<LOGN>::BT, RT = Plan<LOGN>::RT;
     const uint32_t mi = QP ? c.q : c.p, mf = QP ? c.p : c.q;
     inv_pass<LOGN, 0, 3, !QP>(v, t, twi, mi);
@@ -275,6 +275,7 @@ __device__ __forceinline__ void round_trip(uint32_t v[8], uint32_t *X, const SmC
         lds_store<LOGN, BT, RT>(X, u, t);
     }
     pass_sync<LOGN>();
+    mid();  // issued here: loads the caller needs after the transform
     fwd_mid<LOGN, BT - 3, QP>(X, twf, mf, t);
     own_load(X, v, t);
     fwd_pass<LOGN, 0, 3, QP>(v, t, twf, mf);
@@ -373,33 +374,28 @@ __global__ __launch_bounds__((1 << LOGN) / 8, 4) void k_sm_respond(SmCtxDev c, S
             __syncthreads();
         }
     }
-    uint32_t A[8];  // NTT(a), lazy in [0, 4q)
-    forward_from_lds<LOGN, false>(X, A, c.twf_q, q, t);
-
-    // --- the seat's rows on own slots, and the rotation by tau_g as LDS offsets (two
-    // 16-bit offsets per word)
-    uint32_t WQ[8], KQ[8], WP[8], KP[8];
+    // NTT(a), lazy in [0, 4q), parked in the query's slice of the mask output: like the
+    // seat's rows, it is read once per step, just ahead of its use, instead of pinning
+    // registers for all steps
+    uint32_t *apark = bt.mask_out + (size_t)qi * N + 8 * t;
+    uint32_t V[8], BQ[8], BP[8];
     {
-        const uint32_t *s = job.seat + 8 * t;
-        gload8(s, WQ);
-        gload8(s + N, KQ);
-        gload8(s + 2 * N, WP);
-        gload8(s + 3 * N, KP);
+        uint32_t A[8], S[8];
+        forward_from_lds<LOGN, false>(X, A, c.twf_q, q, t);
+        gstore8(apark, A);
+        gload8(job.step1 + (size_t)(c.gamma - 1) * N + 8 * t, S);
+#pragma unroll
+        for (int k = 0; k < 8; k++) V[k] = red(mont(A[k], S[k], q, c.qninv), 2 * q);
     }
+    const uint32_t *seat = job.seat + 8 * t;
+
+    // --- the rotation by tau_g as LDS offsets (two 16-bit offsets per word)
     uint32_t rot[4];
 #pragma unroll
     for (int k = 0; k < 4; k++)
         rot[k] = (uint32_t)swz(rot_slot(c.g, 8 * t + 2 * k, LOGN)) |
                  (uint32_t)swz(rot_slot(c.g, 8 * t + 2 * k + 1, LOGN)) << 16;
 
-    // --- V_{gamma-1} = a * S'_{gamma-1}
-    uint32_t V[8], BQ[8], BP[8];
-    {
-        uint32_t S[8];
-        gload8(job.step1 + (size_t)(c.gamma - 1) * N + 8 * t, S);
-#pragma unroll
-        for (int k = 0; k < 8; k++) V[k] = red(mont(A[k], S[k], q, c.qninv), 2 * q);
-    }
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         BQ[k] = 0;
@@ -422,18 +418,27 @@ __global__ __launch_bounds__((1 << LOGN) / 8, 4) void k_sm_respond(SmCtxDev c, S
             BP[k] = PP[r];
         }
         // tp = NTT_p(lift(iNTT_q(Y)))
-        uint32_t v[8], S[8];
+        uint32_t v[8], S[8], A[8], WQ[8], KQ[8], WP[8], KP[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) v[k] = V[k];
-        round_trip<LOGN, true>(v, X, c, twi_q, twf_p, t);
+        round_trip<LOGN, true>(v, X, c, twi_q, twf_p, t, [&] {
+            const uint32_t *sp = seat + opaque_s(0);
+            gload8(sp + 2 * N, WP);
+            gload8(sp + 3 * N, KP);
+        });
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             BP[k] = red(BP[k] + red(mont(v[k], KP[k], p, c.pninv), p), p);
             v[k] = red(mont(v[k], WP[k], p, c.pninv), p);
         }
         // r = NTT_q(lift(iNTT_p(tp * w_p)))
-        gload8(job.step1 + (size_t)i * N + 8 * t, S);
-        round_trip<LOGN, false>(v, X, c, twi_p, twf_q, t);
+        round_trip<LOGN, false>(v, X, c, twi_p, twf_q, t, [&] {
+            const int z2 = opaque_s(0);
+            gload8(job.step1 + (size_t)i * N + 8 * t + z2, S);
+            gload8(apark + z2, A);
+            gload8(seat + z2, WQ);
+            gload8(seat + N + z2, KQ);
+        });
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             const uint32_t t1 = mont(A[k], S[k], q, c.qninv);
