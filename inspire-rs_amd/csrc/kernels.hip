@@ -146,54 +146,63 @@ __global__ __launch_bounds__(kThreads) void k_column_tags(const uint16_t *col, c
     }
 }
 
-// DB scan. Block (query, split) covers rows [split*n/ks, (split+1)*n/ks) of the
-// query's shard and every column. A lane reads four 16-bit column values of a row
-// with one 8-byte load, so a wave streams whole 512-byte rows; per column it
-// accumulates sum_k brev[k]*h[k] and sum_k key[k]*h[k] as two 32-bit halves in u64
-// (no overflow for n <= 4096). brev is b laid out for coefficient 0 of b*h
-// (reverse_for_coeff0, shard_store.rs:378-391).
+// DB scan, one lane per (row split, 4-column group): the lane walks its split's rows and
+// keeps all 16 partial products of its four columns in registers, so there is no
+// cross-lane reduction and no LDS. A wave's 64 lanes read one row's 64 consecutive u64
+// words (512 contiguous bytes); the row's b and tag-key words are the same on every lane
+// of the wave. Eight rows are in flight per lane.
+// Partials: scan_part[q][split][column][4] = (sum h*b_lo, sum h*b_hi, sum h*key_lo, sum h*key_hi)
+// over the split's rows, 32-bit halves in u64 (below 2^56 for n <= 4096); the chain kernel
+// adds the splits. brev[k] is b laid out for coefficient 0 of b*h (reverse_for_coeff0,
+// shard_store.rs:378-391); the sums are check_columns_and_fold_b's (shard_store.rs:197-265).
+template <bool UNIFORM_ROW>
 __global__ __launch_bounds__(kThreads) void k_scan(CtxDev c, BatchDev bt) {
-    const int qi = blockIdx.x, split = blockIdx.y;
+    const int qi = blockIdx.x;
     const int n = c.n, gamma = c.gamma;
+    const int G = gamma / 4;                     // 4-column groups per row (gamma >= 4)
+    const int per_block = blockDim.x / G;        // splits per block
+    const int g = threadIdx.x % G;
+    int split = blockIdx.y * per_block + threadIdx.x / G;
+    // with 64 or more groups a wave lies inside one split, so the row words are scalar
+    if (UNIFORM_ROW) split = __builtin_amdgcn_readfirstlane(split);
     const ShardDev sh = c.shards[bt.shard_ids[qi]];
     const uint64_t *b = bt.b + (size_t)qi * n;
-    const int rows = n / bt.ks, r0 = split * rows;
-    const int G = gamma / 4;                 // 4-column groups per row (gamma >= 4)
-    const int P = kThreads / G;              // rows in flight per block pass
-    const int g = threadIdx.x % G, ro = threadIdx.x / G;
+    const int rows = n / bt.ks, r0 = split * rows;  // a multiple of 8: n >= 256, ks <= n / 128
+    const uint64_t *rec = reinterpret_cast<const uint64_t *>(sh.records) + g;
     uint64_t acc[4][4];
 #pragma unroll
     for (int u = 0; u < 4; u++) acc[u][0] = acc[u][1] = acc[u][2] = acc[u][3] = 0;
-    const uint64_t *rec = reinterpret_cast<const uint64_t *>(sh.records);
-#pragma unroll 4
-    for (int k = r0 + ro; k < r0 + rows; k += P) {
-        const uint64_t h4 = rec[(size_t)k * G + g];
-        const uint64_t bk = k == 0 ? b[0] : (b[n - k] ? kQ - b[n - k] : 0);
-        const uint64_t key = sh.tag_key[k];
-        const uint32_t bl = (uint32_t)bk, bh = (uint32_t)(bk >> 32);
-        const uint32_t kl = (uint32_t)key, kh = (uint32_t)(key >> 32);
+    for (int k0 = r0; k0 < r0 + rows; k0 += 8) {
+        uint64_t h4[8], bv[8], kv[8];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const uint32_t h = (uint32_t)(h4 >> (16 * u)) & 0xffffu;
-            acc[u][0] += (uint64_t)h * bl;
-            acc[u][1] += (uint64_t)h * bh;
-            acc[u][2] += (uint64_t)h * kl;
-            acc[u][3] += (uint64_t)h * kh;
+        for (int r = 0; r < 8; r++) {
+            const int k = k0 + r;
+            h4[r] = rec[(size_t)k * G];
+            bv[r] = b[(n - k) & (n - 1)];
+            kv[r] = sh.tag_key[k];
+        }
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const int k = k0 + r;
+            // brev[0] = b[0], brev[k] = -b[n-k]
+            const uint64_t bk = (k != 0 && bv[r] != 0) ? kQ - bv[r] : bv[r];
+            const uint32_t bl = (uint32_t)bk, bh = (uint32_t)(bk >> 32);
+            const uint32_t kl = (uint32_t)kv[r], kh = (uint32_t)(kv[r] >> 32);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t h = (uint32_t)(h4[r] >> (16 * u)) & 0xffffu;
+                acc[u][0] += (uint64_t)h * bl;
+                acc[u][1] += (uint64_t)h * bh;
+                acc[u][2] += (uint64_t)h * kl;
+                acc[u][3] += (uint64_t)h * kh;
+            }
         }
     }
-    __shared__ uint64_t red[kThreads][16];
+    uint64_t *o = bt.scan_part + (((size_t)qi * bt.ks + split) * gamma + 4 * g) * 4;
 #pragma unroll
     for (int u = 0; u < 4; u++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) red[threadIdx.x][u * 4 + v] = acc[u][v];
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < G * 16; idx += kThreads) {
-        const int gg = idx / 16, slot = idx % 16;
-        uint64_t s = 0;
-        for (int p = 0; p < P; p++) s += red[p * G + gg][slot];
-        const int col = 4 * gg + slot / 4;
-        bt.scan_part[(((size_t)qi * bt.ks + split) * gamma + col) * 4 + (slot % 4)] = s;
-    }
+        for (int v = 0; v < 4; v++) o[u * 4 + v] = acc[u][v];
 }
 
 // Narrow shards (gamma < 4 columns): one column per lane.
@@ -232,8 +241,15 @@ __global__ __launch_bounds__(kThreads) void k_scan_narrow(CtxDev c, BatchDev bt)
 
 hipError_t launch_scan_only(const CtxDev &c, const BatchDev &b, hipStream_t s) {
     if (b.count == 0) return hipSuccess;
-    if (c.gamma >= 4)
-        hipLaunchKernelGGL(k_scan, dim3(b.count, b.ks), dim3(kThreads), 0, s, c, b);
+    if (c.gamma >= 4) {
+        // one lane per (split, group): a block holds whole splits, the grid the rest
+        const int G = c.gamma / 4;
+        const int threads = G * b.ks < kThreads ? G * b.ks : kThreads;
+        if (G >= 64)
+            hipLaunchKernelGGL(k_scan<true>, dim3(b.count, b.ks / (threads / G)), dim3(threads), 0, s, c, b);
+        else
+            hipLaunchKernelGGL(k_scan<false>, dim3(b.count, b.ks / (threads / G)), dim3(threads), 0, s, c, b);
+    }
     else
         hipLaunchKernelGGL(k_scan_narrow, dim3(b.count, b.ks), dim3(kThreads), 0, s, c, b);
     return hipGetLastError();
