@@ -80,6 +80,20 @@ __device__ __forceinline__ void ct4(uint64_t v[4], int b, int hi, const Pair *__
     ct(v[2], v[3], w2);
 }
 
+// Ordering between two radix-4 passes. Lane t owns slots 4t..4t+3, so each 64-lane wave
+// owns 256 consecutive slots in every pass over bits below 8 (the groups of the pass at
+// base bit b <= 6 are 4^(b/2) consecutive lanes); exchanges between two such passes need
+// only the wave's own LDS order, and a pass touching bit 8 or above takes the barrier.
+__device__ __forceinline__ void step_sync(bool wave_local) {
+    if (wave_local) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 __device__ __forceinline__ void group_of(int g, int b, int &base, int &hi) {
     const int lo = g & ((1 << b) - 1);
     hi = g >> b;
@@ -127,10 +141,11 @@ __device__ __forceinline__ void inverse_to(uint64_t s[4], uint64_t *x, int stop,
     const int base = 4 * threadIdx.x;
 #pragma unroll
     for (int k = 0; k < 4; k++) x[lds_ix(base + k)] = v[k];
-    __syncthreads();
+    step_sync(stop > 2);  // wave-local when the pass over bits 2-3 follows
     for (int b = 2; b < stop; b += 2) {
         lds_inv_pass<LOGN>(x, b, tw);
-        __syncthreads();
+        // the next pass is over bits b+2, b+3, or the top of the transform
+        step_sync(b + 4 <= 8 && b + 2 < stop);
     }
 }
 
@@ -175,7 +190,7 @@ __device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi
         __syncthreads();
         for (int b = LOGN - 3; b >= 2; b -= 2) {
             lds_fwd_pass2<LOGN>(C, D, b, twf);
-            __syncthreads();
+            step_sync(b + 2 <= 8);  // the next pass is over bits b-2, b-1 (or the own slots)
         }
     } else {
         // top two bits: one radix-4 group per thread
@@ -199,7 +214,7 @@ __device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi
 #pragma unroll
         for (int bb = LOGN - 4; bb >= 2; bb -= 2) {
             lds_fwd_pass2<LOGN>(C, D, bb, twf);
-            __syncthreads();
+            step_sync(bb + 2 <= 8);
         }
     }
 }
