@@ -230,6 +230,31 @@ int inspire_sm_respond_batch_device(inspire_sm_server *server, size_t count,
                                     uint32_t *d_body_out, void *stream, float *kernel_ms, char *err,
                                     size_t err_cap);
 
+/* Continuous-batching queue over one special-modulus server: requests (a batch of
+ * queries under one seated key, each query naming its own table) from any thread are
+ * merged into launches of up to max_launch_queries while the previous launch runs; every
+ * query's result equals its Server::respond alone. Keys are seated by handle and admitted
+ * against hbm_budget_bytes (0 = free device memory only). Tables must outlive the
+ * requests that name them. No counterpart type in the reference, which answers each
+ * query on the caller's thread (server.rs:252-279). */
+typedef struct inspire_sm_batcher inspire_sm_batcher;
+int inspire_sm_batcher_create(inspire_sm_server *server, size_t max_launch_queries, size_t hbm_budget_bytes,
+                              inspire_sm_batcher **out, char *err, size_t err_cap);
+void inspire_sm_batcher_destroy(inspire_sm_batcher *batcher);
+/* Server::seat for a key kept by handle. */
+int inspire_sm_batcher_register_key(inspire_sm_batcher *batcher, const uint8_t mask_seed[32], const uint32_t *key_q,
+                                    const uint32_t *key_p, uint64_t *handle_out, char *err, size_t err_cap);
+int inspire_sm_batcher_unregister_key(inspire_sm_batcher *batcher, uint64_t handle, char *err, size_t err_cap);
+int inspire_sm_batcher_submit(inspire_sm_batcher *batcher, uint64_t handle, const inspire_sm_table *const *tables,
+                              const uint8_t *seeds, const uint32_t *bodies, size_t count, uint64_t *ticket_out,
+                              char *err, size_t err_cap);
+/* Blocks until the ticket is answered; outputs as inspire_sm_respond_batch, latency_ms =
+ * submit to completion. */
+int inspire_sm_batcher_wait(inspire_sm_batcher *batcher, uint64_t ticket, uint32_t *mask_out, uint32_t *body_out,
+                            double *latency_ms, char *err, size_t err_cap);
+int inspire_sm_batcher_stats(inspire_sm_batcher *batcher, uint64_t *launches, uint64_t *queries,
+                             uint64_t *resident_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,41 +11,12 @@
 #include <new>
 #include <vector>
 
-#include "../../include/inspire_gpu.h"
-#include "sm.hpp"
+#include "sm_runtime.hpp"
 
 using namespace inspire;
+using namespace inspire_sm;
 
 namespace {
-
-void sm_err(char *err, size_t cap, const char *fmt, ...) {
-    if (!err || cap == 0) return;
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(err, cap, fmt, ap);
-    va_end(ap);
-}
-
-#define SM_TRY(expr)                                                                       \
-    do {                                                                                   \
-        hipError_t e_ = (expr);                                                            \
-        if (e_ != hipSuccess) {                                                            \
-            sm_err(err, err_cap, "HIP failure in %s: %s", #expr, hipGetErrorString(e_));   \
-            return INSPIRE_ERR_DEVICE;                                                     \
-        }                                                                                  \
-    } while (0)
-
-struct Buf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    ~Buf() { if (p) (void)hipFree(p); }
-    hipError_t alloc(size_t b) {
-        if (p) { (void)hipFree(p); p = nullptr; }
-        bytes = b;
-        return hipMalloc(&p, b ? b : 8);
-    }
-    template <class T> T *as() const { return static_cast<T *>(p); }
-};
 
 uint64_t pw(uint64_t b, uint64_t e, uint64_t m) {
     uint64_t r = 1;
@@ -91,31 +62,6 @@ uint64_t first_root(uint64_t m, uint64_t n) {
 }
 
 }  // namespace
-
-struct inspire_sm_server {
-    std::mutex mu;
-    inspire_sm_params params{};
-    SmCtxDev dev{};
-    Buf tw, cols;
-    // staging of the host-buffer entry points
-    Buf jobs, seeds, bodies, mask, body;
-    size_t cap = 0;
-    // job list of the device entry point, reused while the same handles repeat
-    Buf djobs;
-    std::vector<SmJob> djobs_host;
-    size_t djobs_cap = 0;
-};
-
-struct inspire_sm_table {
-    inspire_sm_server *srv = nullptr;
-    Buf step1, cells;
-    uint32_t rows = 0;
-};
-
-struct inspire_sm_seat {
-    inspire_sm_server *srv = nullptr;
-    Buf words;
-};
 
 extern "C" {
 

@@ -1,0 +1,70 @@
+// Internal runtime objects of the special-modulus set, shared by the C ABI (sm_capi.cpp)
+// and the continuous-batching queue (sm_batcher.cpp). Not part of the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <mutex>
+#include <vector>
+
+#include "../../include/inspire_gpu.h"
+#include "sm.hpp"
+
+namespace inspire_sm {
+
+inline void sm_err(char *err, size_t cap, const char *fmt, ...) {
+    if (!err || cap == 0) return;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(err, cap, fmt, ap);
+    va_end(ap);
+}
+
+#define SM_TRY(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            inspire_sm::sm_err(err, err_cap, "HIP failure in %s: %s", #expr, hipGetErrorString(e_)); \
+            return INSPIRE_ERR_DEVICE;                                                     \
+        }                                                                                  \
+    } while (0)
+
+struct Buf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~Buf() { if (p) (void)hipFree(p); }
+    hipError_t alloc(size_t b) {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        bytes = b;
+        return hipMalloc(&p, b ? b : 8);
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+}  // namespace inspire_sm
+
+struct inspire_sm_server {
+    std::mutex mu;
+    inspire_sm_params params{};
+    inspire::SmCtxDev dev{};
+    inspire_sm::Buf tw, cols;
+    // staging of the host-buffer entry points
+    inspire_sm::Buf jobs, seeds, bodies, mask, body;
+    size_t cap = 0;
+    // job list of the device entry point, reused while the same handles repeat
+    inspire_sm::Buf djobs;
+    std::vector<inspire::SmJob> djobs_host;
+    size_t djobs_cap = 0;
+};
+
+struct inspire_sm_table {
+    inspire_sm_server *srv = nullptr;
+    inspire_sm::Buf step1, cells;
+    uint32_t rows = 0;
+};
+
+struct inspire_sm_seat {
+    inspire_sm_server *srv = nullptr;
+    inspire_sm::Buf words;
+};

@@ -111,6 +111,13 @@ def lib():
         L.inspire_sm_respond.argtypes = [vp, vp, vp, vp, vp, vp, vp, cp, sz]
         L.inspire_sm_respond_batch.argtypes = [vp, sz, vp, vp, vp, vp, vp, vp, cp, sz]
         L.inspire_sm_respond_batch_device.argtypes = [vp, sz, vp, vp, vp, vp, vp, vp, vp, vp, cp, sz]
+        L.inspire_sm_batcher_create.argtypes = [vp, sz, sz, pp, cp, sz]
+        L.inspire_sm_batcher_destroy.argtypes = [vp]
+        L.inspire_sm_batcher_register_key.argtypes = [vp, vp, vp, vp, ctypes.POINTER(u64), cp, sz]
+        L.inspire_sm_batcher_unregister_key.argtypes = [vp, u64, cp, sz]
+        L.inspire_sm_batcher_submit.argtypes = [vp, u64, vp, vp, vp, sz, ctypes.POINTER(u64), cp, sz]
+        L.inspire_sm_batcher_wait.argtypes = [vp, u64, vp, vp, ctypes.POINTER(ctypes.c_double), cp, sz]
+        L.inspire_sm_batcher_stats.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64)]
         _lib = L
     return _lib
 
@@ -638,3 +645,58 @@ class SmServer:
               ctypes.c_void_p(d_bodies), ctypes.c_void_p(d_mask), ctypes.c_void_p(d_body), ctypes.c_void_p(stream),
               ctypes.byref(ms) if timed else None)
         return ms.value if timed else None
+
+
+class SmBatcher:
+    """Continuous-batching queue over one SmServer: submit() from any thread, wait() for the
+    (masks, bodies, latency_ms) of a ticket."""
+
+    def __init__(self, server, max_launch=2048, hbm_budget=0):
+        h = ctypes.c_void_p()
+        _call(lib().inspire_sm_batcher_create, server._h, max_launch, hbm_budget, ctypes.byref(h))
+        self._h, self.server = h, server
+        self.n = server.params.ring_dim
+        self.nb = server.params.body_coefficients()
+        self._counts = {}
+        self._lock = __import__("threading").Lock()
+
+    def register_key(self, key):
+        seed = np.frombuffer(key.mask_seed, dtype=np.uint8).copy()
+        h = ctypes.c_uint64()
+        _call(lib().inspire_sm_batcher_register_key, self._h, _ptr(seed), _ptr(key.q), _ptr(key.p), ctypes.byref(h))
+        return h.value
+
+    def unregister_key(self, handle):
+        _call(lib().inspire_sm_batcher_unregister_key, self._h, handle)
+
+    def submit(self, handle, tables, queries):
+        count = len(queries)
+        th = (ctypes.c_void_p * count)(*[t._h.value for t in tables])
+        seeds = np.frombuffer(b"".join(qy.seed for qy in queries), dtype=np.uint8).copy()
+        bodies = np.ascontiguousarray(np.stack([qy.body for qy in queries]), dtype=np.uint32)
+        if bodies.shape[1] != self.n:
+            raise PirError("query does not fit the set")
+        t = ctypes.c_uint64()
+        _call(lib().inspire_sm_batcher_submit, self._h, handle, th, _ptr(seeds), _ptr(bodies), count, ctypes.byref(t))
+        with self._lock:
+            self._counts[t.value] = count
+        return t.value
+
+    def wait(self, ticket):
+        with self._lock:
+            count = self._counts.pop(ticket)
+        masks = np.zeros((count, self.n), np.uint32)
+        bodies = np.zeros((count, self.nb), np.uint32)
+        lat = ctypes.c_double()
+        _call(lib().inspire_sm_batcher_wait, self._h, ticket, _ptr(masks), _ptr(bodies), ctypes.byref(lat))
+        return masks, bodies, lat.value
+
+    def stats(self):
+        a, q, r = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        lib().inspire_sm_batcher_stats(self._h, ctypes.byref(a), ctypes.byref(q), ctypes.byref(r))
+        return {"launches": a.value, "queries": q.value, "resident_bytes": r.value}
+
+    def close(self):
+        if self._h:
+            lib().inspire_sm_batcher_destroy(self._h)
+            self._h = None
