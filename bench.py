@@ -339,6 +339,8 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--merge", type=int, default=8, help="64-query batches merged into one launch")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="time stream launches instead of hipGraph replays")
+    ap.add_argument("--no-autotune", action="store_true", help="keep --queries-per-cu instead of autotuning")
     ap.add_argument("--served-seconds", type=float, default=4.0)
     ap.add_argument("--queries-per-cu", type=int, default=2, help="launch shaping of the packing kernel (1 or 2)")
     ap.add_argument("--shards", type=int, default=128)
@@ -412,6 +414,12 @@ def main():
         ctx.respond_batch_device(seat, ws, d_ids[i].data_ptr(), d_seeds[i].data_ptr(), d_b[i].data_ptr(), L,
                                  d_a_out.data_ptr(), d_b_out.data_ptr(), stream=stream, switch_modulus=Q36)
 
+    tuned = None
+    if not args.no_autotune:
+        # launch shape picked by timing both shapes on synthetic queries at this launch size
+        tuned = g.autotune(ctx, seat, ws, L)
+        log(f"bench: autotune at {L} queries/launch: {tuned}")
+
     log("bench: warmup")
     for i in range(args.warmup):
         issue(i)
@@ -419,23 +427,45 @@ def main():
     st_ = ws.status(L)
     assert (st_ == 0).all(), f"respond status {st_}"
 
-    # timed region: `steps` launches back to back on one stream, each answering `merge`
-    # batches of B queries; a batch's latency is its launch's (events on the stream)
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(launches):
-        starts[k].record()
-        issue(args.warmup + k)
-        ends[k].record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    lat = np.array([s_.elapsed_time(e) for s_, e in zip(starts, ends)])
+    def timed_launches(launch):
+        # `steps` launches back to back on one stream, each answering `merge` batches of B
+        # queries; a batch's latency is its launch's (events on the stream)
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(launches):
+            starts[k].record()
+            launch(k)
+            ends[k].record()
+        torch.cuda.synchronize()
+        wall_ = time.perf_counter() - t0
+        st = ws.status(L)
+        assert (st == 0).all(), f"respond status {st}"
+        return wall_, np.array([s_.elapsed_time(e) for s_, e in zip(starts, ends)])
+
+    eager_wall, eager_lat = timed_launches(lambda k: issue(args.warmup + k))
+    eager_last = d_a_out.clone()
+    pipeline, wall, lat = "stream", eager_wall, eager_lat
+    if not args.no_graph:
+        # the pipeline captured once per staged batch (kernel nodes only), replayed
+        log("bench: hipGraph-captured pipeline")
+        graphs = [g.RespondGraph(ctx, seat, ws, d_ids[args.warmup + k].data_ptr(),
+                                 d_seeds[args.warmup + k].data_ptr(), d_b[args.warmup + k].data_ptr(), L,
+                                 d_a_out.data_ptr(), d_b_out.data_ptr(), switch_modulus=Q36)
+                  for k in range(launches)]
+        for gr in graphs:
+            gr.launch(stream)
+        torch.cuda.synchronize()
+        d_a_out.zero_()
+        pipeline = "hipGraph"
+        wall, lat = timed_launches(lambda k: graphs[k].launch(stream))
+        assert torch.equal(d_a_out, eager_last), "graph replay differs from the stream launch"
+        for gr in graphs:
+            gr.close()
     ms_per_step = wall * 1000 / (launches * M)
     qps = L * launches / wall
-    st_ = ws.status(L)
-    assert (st_ == 0).all(), f"respond status {st_}"
+    eager_qps = L * launches / eager_wall
 
     def step(i):
         return g.respond_batch_device_timed(ctx, seat, ws, d_ids[i].data_ptr(), d_seeds[i].data_ptr(),
@@ -531,7 +561,9 @@ def main():
                         f"36-bit modulus switch, {M} batches merged per launch ({L} queries), launches back to back",
             "merge": M,
             "queries_per_launch": L,
-            "queries_per_cu": args.queries_per_cu,
+            "pipeline": pipeline,
+            "queries_per_cu": tuned["queries_per_cu"] if tuned else args.queries_per_cu,
+            "autotune": tuned,
             "global_batch": B,
             "records": args.shards * N,
             "record_bytes": ENTRY,
@@ -539,6 +571,9 @@ def main():
         "db_gbs": round(qps * N * ENTRY / 1e9, 3),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3)},
+        "stream_launches": {"value": round(eager_qps, 2), "unit": "queries/s",
+                            "latency_ms": {"p50": round(float(np.percentile(eager_lat, 50)), 3),
+                                           "p99": round(float(np.percentile(eager_lat, 99)), 3)}},
         "plain_gpu": {"value": round(plain_qps, 2), "unit": "queries/s",
                       "workload": "one seeded query per launch, 2^16 records x 512 B (32 shards)"},
         "vs_plain_gpu": round(qps / plain_qps, 3),

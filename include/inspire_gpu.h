@@ -137,6 +137,26 @@ int inspire_respond_batch_device_timed(inspire_ctx *ctx, const inspire_seat *sea
 /* Copies count statuses (0 = answered, c+1 = column c failed its store tag). */
 int inspire_workspace_status(inspire_workspace *ws, size_t count, int32_t *status, char *err, size_t err_cap);
 
+/* The respond pipeline of inspire_respond_batch_device (DB scan + packing chain) captured
+ * once into a hipGraph over fixed device buffers: a serving loop rewrites the input
+ * buffers and replays it with one launch. The graph holds kernel nodes only. It stays
+ * valid while the context's set of shards is unchanged. No reference counterpart
+ * (results are those of inspire_respond_batch_device). */
+typedef struct inspire_graph inspire_graph;
+int inspire_respond_graph_create(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws,
+                                 const uint32_t *d_shard_ids, const uint8_t *d_seeds, const uint64_t *d_b,
+                                 size_t count, uint64_t switch_modulus, uint64_t *d_a_out, uint64_t *d_b_prefix_out,
+                                 inspire_graph **out, char *err, size_t err_cap);
+int inspire_graph_launch(inspire_graph *graph, void *stream, char *err, size_t err_cap);
+void inspire_graph_destroy(inspire_graph *graph);
+
+/* Launch-shape autotuner (no reference counterpart; results are unchanged): times the
+ * respond pipeline on synthetic queries of the real shape (count queries over the
+ * context's stored shards) at 1 and 2 queries per CU, keeps the faster shape in the
+ * context, and reports both times in kernel_ms and the choice in *chosen. */
+int inspire_ctx_autotune(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws, size_t count,
+                         float kernel_ms[2], uint32_t *chosen, char *err, size_t err_cap);
+
 /* Continuous-batching server over one context (the serving side of
  * respond_seeded_batch with registered keys, src/pir/respond.rs:1189, and of
  * ServerSessionStore, src/pir/session.rs:700-896). A worker thread merges queued

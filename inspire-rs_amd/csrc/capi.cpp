@@ -469,6 +469,141 @@ int inspire_respond_batch_device_timed(inspire_ctx *ctx, const inspire_seat *sea
                           stream, kernel_ms, err, err_cap);
 }
 
+struct inspire_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~inspire_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+int inspire_respond_graph_create(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws,
+                                 const uint32_t *d_shard_ids, const uint8_t *d_seeds, const uint64_t *d_b,
+                                 size_t count, uint64_t switch_modulus, uint64_t *d_a_out, uint64_t *d_b_prefix_out,
+                                 inspire_graph **out, char *err, size_t err_cap) {
+    if (!out) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    if (!ctx || !ws || ws->ctx != ctx) { set_err(err, err_cap, "workspace belongs to another context"); return INSPIRE_ERR_INVALID; }
+    int rc = check_batch(ctx, seat, nullptr, count, err, err_cap);
+    if (rc) return rc;
+    if (count == 0 || count > ws->max_count) {
+        set_err(err, err_cap, "batch of %zu queries does not fit the workspace's %zu", count, ws->max_count);
+        return INSPIRE_ERR_INVALID;
+    }
+    if (switch_modulus >= (1ULL << 37)) {
+        set_err(err, err_cap, "switch modulus %llu is above the 37-bit targets this path implements",
+                (unsigned long long)switch_modulus);
+        return INSPIRE_ERR_INVALID;
+    }
+    CtxDev dev;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        rc = sync_table(ctx, err, err_cap);
+        if (rc) return rc;
+        dev = ctx->dev();
+    }
+    BatchDev b;
+    b.count = (int)count;
+    b.shard_ids = d_shard_ids;
+    b.seeds = d_seeds;
+    b.yprimes = nullptr;
+    b.a_in = nullptr;
+    b.b = d_b;
+    b.yprime = seat->yprime.as<Pair>();
+    b.scan_part = ws->scan_part.as<uint64_t>();
+    b.ks = ctx->ks();
+    b.a_out = d_a_out;
+    b.b_out = d_b_prefix_out;
+    b.status = ws->status.as<int32_t>();
+    b.switch_modulus = switch_modulus;
+    inspire_graph *g = new (std::nothrow) inspire_graph();
+    if (!g) { set_err(err, err_cap, "out of host memory"); return INSPIRE_ERR_DEVICE; }
+    // the two kernels of the pipeline, captured on a private stream (no memset or copy nodes)
+    hipStream_t cs = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipError_t le = launch_respond(dev, b, cs);
+        e = hipStreamEndCapture(cs, &g->graph);
+        if (le != hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (cs) (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) {
+        set_err(err, err_cap, "HIP failure capturing the respond graph: %s", hipGetErrorString(e));
+        delete g;
+        return INSPIRE_ERR_DEVICE;
+    }
+    *out = g;
+    return INSPIRE_OK;
+}
+
+int inspire_graph_launch(inspire_graph *g, void *stream, char *err, size_t err_cap) {
+    if (!g) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    HIP_TRY(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
+    return INSPIRE_OK;
+}
+
+void inspire_graph_destroy(inspire_graph *g) { delete g; }
+
+int inspire_ctx_autotune(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws, size_t count,
+                         float kernel_ms[2], uint32_t *chosen, char *err, size_t err_cap) {
+    if (!ctx || !ws || ws->ctx != ctx || !kernel_ms || !chosen) {
+        set_err(err, err_cap, "null argument or workspace of another context");
+        return INSPIRE_ERR_INVALID;
+    }
+    if (count == 0 || count > ws->max_count) {
+        set_err(err, err_cap, "batch of %zu queries does not fit the workspace's %zu", count, ws->max_count);
+        return INSPIRE_ERR_INVALID;
+    }
+    std::vector<uint32_t> ids;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        std::vector<uint32_t> stored;
+        for (size_t i = 0; i < ctx->owners.size(); i++)
+            if (ctx->owners[i] && ctx->owners[i]->has_store) stored.push_back((uint32_t)i);
+        if (stored.empty()) {
+            set_err(err, err_cap, "autotune needs a shard with its packing store built");
+            return INSPIRE_ERR_INVALID;
+        }
+        for (size_t k = 0; k < count; k++) ids.push_back(stored[k % stored.size()]);
+    }
+    // synthetic queries of the real shape: distinct seeds, b uniform-looking below q
+    const size_t n = ctx->n, g = ctx->gamma;
+    std::vector<uint8_t> seeds(count * 32);
+    std::vector<uint64_t> b(count * n);
+    uint64_t x = 0x9e3779b97f4a7c15ULL;
+    for (auto &v : seeds) { x ^= x << 13; x ^= x >> 7; x ^= x << 17; v = (uint8_t)x; }
+    for (auto &v : b) { x ^= x << 13; x ^= x >> 7; x ^= x << 17; v = x % kQ; }
+    DevBuf d_ids, d_seeds, d_b, d_a, d_bp;
+    HIP_TRY(d_ids.alloc(count * 4));
+    HIP_TRY(d_seeds.alloc(count * 32));
+    HIP_TRY(d_b.alloc(count * n * 8));
+    HIP_TRY(d_a.alloc(count * n * 8));
+    HIP_TRY(d_bp.alloc(count * g * 8));
+    HIP_TRY(hipMemcpy(d_ids.p, ids.data(), count * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_seeds.p, seeds.data(), count * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_b.p, b.data(), count * n * 8, hipMemcpyHostToDevice));
+    const int prev = ctx->queries_per_cu;
+    float best = 0;
+    uint32_t pick = (uint32_t)prev;
+    for (int qpc = 1; qpc <= 2; qpc++) {
+        ctx->queries_per_cu = qpc;
+        float t[2] = {0, 0}, sum = 0;
+        for (int rep = 0; rep < 3; rep++) {  // one warm-up, two timed
+            int rc = respond_device(ctx, seat, ws, d_ids.as<uint32_t>(), d_seeds.as<uint8_t>(), d_b.as<uint64_t>(),
+                                    count, 0, d_a.as<uint64_t>(), d_bp.as<uint64_t>(), nullptr, t, err, err_cap);
+            if (rc) { ctx->queries_per_cu = prev; return rc; }
+            if (rep) sum += t[0] + t[1];
+        }
+        kernel_ms[qpc - 1] = sum / 2;
+        if (qpc == 1 || sum / 2 < best) { best = sum / 2; pick = (uint32_t)qpc; }
+    }
+    ctx->queries_per_cu = (int)pick;
+    *chosen = pick;
+    return INSPIRE_OK;
+}
+
 int inspire_workspace_status(inspire_workspace *ws, size_t count, int32_t *status, char *err, size_t err_cap) {
     if (!ws || !status || count > ws->max_count) { set_err(err, err_cap, "bad status request"); return INSPIRE_ERR_INVALID; }
     HIP_TRY(hipMemcpy(status, ws->status.p, count * sizeof(int32_t), hipMemcpyDeviceToHost));

@@ -94,6 +94,10 @@ def lib():
         L.inspire_server_wait.argtypes = [vp, u64, vp, vp, ctypes.POINTER(ctypes.c_double), cp, sz]
         L.inspire_server_stats.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.inspire_workspace_status.argtypes = [vp, sz, vp, cp, sz]
+        L.inspire_respond_graph_create.argtypes = [vp, vp, vp, vp, vp, vp, sz, u64, vp, vp, pp, cp, sz]
+        L.inspire_graph_launch.argtypes = [vp, vp, cp, sz]
+        L.inspire_graph_destroy.argtypes = [vp]
+        L.inspire_ctx_autotune.argtypes = [vp, vp, vp, sz, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(u32), cp, sz]
         smp = ctypes.POINTER(SmParams)
         L.inspire_sm_params_validate.argtypes = [smp, cp, sz]
         L.inspire_sm_server_create.argtypes = [smp, pp, cp, sz]
@@ -288,6 +292,34 @@ class Context:
         _call(lib().inspire_respond_batch_device, self._h, seat._h, ws._h, ctypes.c_void_p(d_ids),
               ctypes.c_void_p(d_seeds), ctypes.c_void_p(d_b), count, switch_modulus, ctypes.c_void_p(d_a_out),
               ctypes.c_void_p(d_b_out), ctypes.c_void_p(stream))
+
+
+class RespondGraph:
+    """The respond pipeline captured once into a hipGraph over fixed device buffers; launch()
+    replays it on a stream."""
+
+    def __init__(self, ctx, seat, ws, d_ids, d_seeds, d_b, count, d_a_out, d_b_out, switch_modulus=0):
+        h = ctypes.c_void_p()
+        _call(lib().inspire_respond_graph_create, ctx._h, seat._h, ws._h, ctypes.c_void_p(d_ids),
+              ctypes.c_void_p(d_seeds), ctypes.c_void_p(d_b), count, switch_modulus, ctypes.c_void_p(d_a_out),
+              ctypes.c_void_p(d_b_out), ctypes.byref(h))
+        self._h = h
+
+    def launch(self, stream=0):
+        _call(lib().inspire_graph_launch, self._h, ctypes.c_void_p(stream))
+
+    def close(self):
+        if self._h:
+            lib().inspire_graph_destroy(self._h)
+            self._h = None
+
+
+def autotune(ctx, seat, ws, count):
+    """inspire_ctx_autotune: times 1 and 2 queries per CU at `count` queries, keeps the faster."""
+    ms = (ctypes.c_float * 2)()
+    chosen = ctypes.c_uint32()
+    _call(lib().inspire_ctx_autotune, ctx._h, seat._h, ws._h, count, ms, ctypes.byref(chosen))
+    return {"queries_per_cu": chosen.value, "ms_qpc1": round(ms[0], 4), "ms_qpc2": round(ms[1], 4)}
 
 
 class Server:
