@@ -23,6 +23,10 @@
  *       FNV-1a 0x796a80ca60d31a52 (tests/respond_byte_identity_kat.rs:108-145).
  *   store path -> equality with the no-store path (an invariant the reference
  *       tests too, tests/shard_packing_store.rs:163-208).
+ *   client side of the KAT cells (client.c: secret, packing keys, seeded queries,
+ *       database bytes) -> client_packing_key_kat.txt and the SESSIONKEY lines of
+ *       packing_seat_response_kat.txt; the seeded d2048 and d256 responds -> its RESP
+ *       lines (tests/packing_seat_byte_identity_kat.rs:48-146).
  *   approximate gadget -> decode correctness only: "parity unpinned" at the
  *       byte level (no committed reference vector exists).
  *   mod switch -> u128 closed form brute force.
@@ -152,6 +156,21 @@ size_t orc_encode_response(size_t n, uint64_t modulus, const uint64_t *a_hat, co
 
 /* query a from its seed (Poly::from_seed_moduli, poly.rs:682 -> :657) */
 void orc_expand_seed(const uint8_t seed[32], size_t n, uint64_t q, uint64_t *a);
+
+/* ---------------- client side of the packing-key KAT cells (client.c) ---------------- */
+typedef struct orc_cell orc_cell;
+void orc_sample_poly(orc_gauss *g, size_t n, uint64_t q, uint64_t *out);
+void orc_apply_automorphism(const uint64_t *in, size_t n, size_t t, uint64_t q, uint64_t *out);
+int orc_client_packing_keys(size_t n, size_t gamma, const uint64_t *s, const uint8_t w_seed[32], orc_gauss *g,
+                            uint64_t *y_body);
+size_t orc_packing_keys_bytes(size_t n, const uint64_t *y_body, size_t rows, size_t gamma, uint8_t *out);
+int orc_client_key_kat(size_t n, size_t gamma, uint64_t seed, uint64_t *y_body, int64_t *next);
+orc_cell *orc_cell_new(size_t n, size_t entry, uint64_t seed);
+void orc_cell_free(orc_cell *c);
+void orc_cell_info(const orc_cell *c, uint8_t w_seed[32], uint64_t *y_body, uint8_t *db, uint64_t *secret);
+void orc_cell_columns(const orc_cell *c, uint32_t shard, uint64_t *columns);
+void orc_cell_query(orc_cell *c, uint64_t index, uint32_t *shard, uint8_t seed[32], uint64_t *b);
+size_t orc_cell_gamma(const orc_cell *c);
 
 /* ---------------- special modulus (src/special_modulus, special.c) ---------------- */
 typedef struct { uint32_t q, neg_qinv, r2; } sm_prime;

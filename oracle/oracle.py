@@ -414,3 +414,105 @@ def _sm_tables(ring):
 
 
 SmRing.tables = _sm_tables
+
+
+# ---------------- client side of the packing-key KAT cells (client.c) ----------------
+
+_lib.orc_client_key_kat.argtypes = [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_uint64, _u64p,
+                                    ctypes.POINTER(ctypes.c_int64)]
+_lib.orc_packing_keys_bytes.argtypes = [ctypes.c_size_t, _u64p, ctypes.c_size_t, ctypes.c_size_t, _u8p]
+_lib.orc_packing_keys_bytes.restype = ctypes.c_size_t
+_lib.orc_cell_new.argtypes = [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_uint64]
+_lib.orc_cell_new.restype = ctypes.c_void_p
+_lib.orc_cell_free.argtypes = [ctypes.c_void_p]
+_lib.orc_cell_info.argtypes = [ctypes.c_void_p, _u8p, _u64p, _u8p, _u64p]
+_lib.orc_cell_columns.argtypes = [ctypes.c_void_p, ctypes.c_uint32, _u64p]
+_lib.orc_cell_query.argtypes = [ctypes.c_void_p, ctypes.c_uint64, _u32p, _u8p, _u64p]
+_lib.orc_cell_gamma.argtypes = [ctypes.c_void_p]
+_lib.orc_cell_gamma.restype = ctypes.c_size_t
+_lib.orc_apply_automorphism.argtypes = [_u64p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_uint64, _u64p]
+
+
+def packing_keys_bytes(y_body, gamma):
+    """bincode of ClientPackingKeys (exact gadget, coefficient-form rows)."""
+    y = np.ascontiguousarray(y_body, dtype=np.uint64)
+    rows, n = y.shape
+    size = _lib.orc_packing_keys_bytes(n, _p64(y), rows, gamma, None)
+    out = (ctypes.c_uint8 * size)()
+    _lib.orc_packing_keys_bytes(n, _p64(y), rows, gamma, ctypes.cast(out, _u8p))
+    return bytes(out)
+
+
+def client_key_kat(n, gamma, seed):
+    """One client_packing_key_kat cell: (key bytes, the sampler's next sample)."""
+    y = np.zeros((3, n), np.uint64)
+    nxt = ctypes.c_int64()
+    if _lib.orc_client_key_kat(n, gamma, seed, _p64(y), ctypes.byref(nxt)) != 0:
+        raise ValueError(f"width {gamma} is not legal at ring {n}")
+    return packing_keys_bytes(y, gamma), nxt.value
+
+
+def apply_automorphism(poly, t, q=Q):
+    a = np.ascontiguousarray(poly, dtype=np.uint64)
+    out = np.zeros_like(a)
+    _lib.orc_apply_automorphism(_p64(a), a.size, t, q, _p64(out))
+    return out
+
+
+class Cell:
+    """A packing-key KAT cell (three shards, one client session, a reproducible query
+    stream), built in the reference fixture's draw order."""
+
+    def __init__(self, n, entry, seed):
+        self._c = _lib.orc_cell_new(n, entry, seed)
+        if not self._c:
+            raise ValueError(f"no cell at ring {n}, entry {entry}")
+        self.n, self.entry, self.seed = n, entry, seed
+        self.gamma = _lib.orc_cell_gamma(self._c)
+        w = (ctypes.c_uint8 * 32)()
+        self.y_body = np.zeros((3, n), np.uint64)
+        self.db = np.zeros(3 * n * entry, np.uint8)
+        self.secret = np.zeros(n, np.uint64)
+        _lib.orc_cell_info(self._c, ctypes.cast(w, _u8p), _p64(self.y_body),
+                           self.db.ctypes.data_as(_u8p), _p64(self.secret))
+        self.w_seed = bytes(w)
+
+    def __del__(self):
+        if getattr(self, "_c", None):
+            _lib.orc_cell_free(self._c)
+            self._c = None
+
+    def key_bytes(self):
+        return packing_keys_bytes(self.y_body, self.gamma)
+
+    def columns(self, shard):
+        cols = np.zeros((self.gamma, self.n), np.uint64)
+        _lib.orc_cell_columns(self._c, shard, _p64(cols))
+        return cols
+
+    def query(self, index):
+        """next_query(index): (shard id, row seed, b)."""
+        shard = ctypes.c_uint32()
+        seed = (ctypes.c_uint8 * 32)()
+        b = np.zeros(self.n, np.uint64)
+        _lib.orc_cell_query(self._c, index, ctypes.byref(shard), ctypes.cast(seed, _u8p), _p64(b))
+        return shard.value, bytes(seed), b
+
+
+def load_packing_seat_kat(path):
+    """Parse packing_seat_response_kat.txt: {(tag, entry, seed): {"key": (len, fnv),
+    "resp": {index: (len, fnv)}}}."""
+    cells = {}
+    for line in open(path):
+        f = line.split()
+        if not f:
+            continue
+        kv = dict(x.split("=", 1) for x in f[2:])
+        key = (f[1], int(kv["entry"]), int(kv["seed"], 16))
+        c = cells.setdefault(key, {"key": None, "resp": {}})
+        rec = (int(kv["len"]), int(kv["fnv"], 16))
+        if f[0] == "SESSIONKEY":
+            c["key"] = rec
+        else:
+            c["resp"][int(kv["index"])] = rec
+    return cells
