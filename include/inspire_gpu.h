@@ -137,6 +137,23 @@ int inspire_respond_batch_device_timed(inspire_ctx *ctx, const inspire_seat *sea
 /* Copies count statuses (0 = answered, c+1 = column c failed its store tag). */
 int inspire_workspace_status(inspire_workspace *ws, size_t count, int32_t *status, char *err, size_t err_cap);
 
+/* ServerResponse::to_binary bytes straight from the device (respond.rs:387, the bincode
+ * layout of respond.rs:111-176 with pack_coefficients_tight, poly.rs:166-225): the packing
+ * kernel's epilogue switches the modulus (when asked) and packs a_hat and the b prefix at
+ * the modulus' bit width into the response body. inspire_response_wire_bytes gives the
+ * length (10,446 B at gamma 256 switched to 36 bits; 17,358 B unswitched).
+ * _device: responses at d_wire + k * wire_stride; d_wire 4-byte aligned, wire_stride a
+ * multiple of 4 and at least the length rounded up to 4 (the pad bytes are written as 0).
+ * Host form: responses at wire_out + k * wire_stride, wire_stride >= the length. */
+size_t inspire_response_wire_bytes(const inspire_ctx *ctx, uint64_t switch_modulus);
+int inspire_respond_batch_device_wire(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws,
+                                      const uint32_t *d_shard_ids, const uint8_t *d_seeds, const uint64_t *d_b,
+                                      size_t count, uint64_t switch_modulus, uint8_t *d_wire, size_t wire_stride,
+                                      void *stream, char *err, size_t err_cap);
+int inspire_respond_batch_wire(inspire_ctx *ctx, const inspire_seat *seat, const uint32_t *shard_ids,
+                               const uint8_t *seeds, const uint64_t *b, size_t count, uint64_t switch_modulus,
+                               uint8_t *wire_out, size_t wire_stride, char *err, size_t err_cap);
+
 /* The respond pipeline of inspire_respond_batch_device (DB scan + packing chain) captured
  * once into a hipGraph over fixed device buffers: a serving loop rewrites the input
  * buffers and replays it with one launch. The graph holds kernel nodes only. It stays

@@ -398,7 +398,8 @@ void inspire_workspace_destroy(inspire_workspace *ws) {
 static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws,
                           const uint32_t *d_shard_ids, const uint8_t *d_seeds, const uint64_t *d_b, size_t count,
                           uint64_t switch_modulus, uint64_t *d_a_out, uint64_t *d_b_prefix_out, void *stream,
-                          float *timing, char *err, size_t err_cap) {
+                          float *timing, char *err, size_t err_cap, uint8_t *d_wire = nullptr,
+                          size_t wire_stride = 0) {
     if (!ctx || !ws || ws->ctx != ctx) { set_err(err, err_cap, "workspace belongs to another context"); return INSPIRE_ERR_INVALID; }
     int rc = check_batch(ctx, seat, nullptr, count, err, err_cap);
     if (rc) return rc;
@@ -430,6 +431,21 @@ static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_wo
     b.b_out = d_b_prefix_out;
     b.status = ws->status.as<int32_t>();
     b.switch_modulus = switch_modulus;
+    if (d_wire) {
+        const WireLayout wl = wire_layout((uint32_t)ctx->n, (uint32_t)ctx->gamma, switch_modulus ? switch_modulus : kQ);
+        if (wire_stride % 4 || wire_stride < ((wl.total + 3) & ~3u) || reinterpret_cast<uintptr_t>(d_wire) % 4) {
+            set_err(err, err_cap,
+                    "response buffer must be 4-byte aligned with a stride that is a multiple of 4 and at least %u "
+                    "(the %u-byte response rounded up)",
+                    (wl.total + 3) & ~3u, wl.total);
+            return INSPIRE_ERR_INVALID;
+        }
+        b.wire_out = d_wire;
+        b.wire_stride = wire_stride;
+    } else if (!d_a_out || !d_b_prefix_out) {
+        set_err(err, err_cap, "null argument");
+        return INSPIRE_ERR_INVALID;
+    }
     if (!timing) {
         HIP_TRY(launch_respond(ctx->dev(), b, static_cast<hipStream_t>(stream)));
         return INSPIRE_OK;
@@ -467,6 +483,33 @@ int inspire_respond_batch_device_timed(inspire_ctx *ctx, const inspire_seat *sea
     if (!kernel_ms) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
     return respond_device(ctx, seat, ws, d_shard_ids, d_seeds, d_b, count, switch_modulus, d_a_out, d_b_prefix_out,
                           stream, kernel_ms, err, err_cap);
+}
+
+static int respond_host(inspire_ctx *ctx, const inspire_seat *seat, const uint32_t *shard_ids, const uint8_t *seeds,
+                        const uint64_t *a_in, const uint64_t *b, size_t count, uint64_t switch_modulus,
+                        uint64_t *a_out, uint64_t *b_prefix_out, char *err, size_t err_cap,
+                        uint8_t *wire_out = nullptr, size_t wire_stride = 0);
+
+size_t inspire_response_wire_bytes(const inspire_ctx *ctx, uint64_t switch_modulus) {
+    if (!ctx) return 0;
+    return wire_layout((uint32_t)ctx->n, (uint32_t)ctx->gamma, switch_modulus ? switch_modulus : kQ).total;
+}
+
+int inspire_respond_batch_device_wire(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws,
+                                      const uint32_t *d_shard_ids, const uint8_t *d_seeds, const uint64_t *d_b,
+                                      size_t count, uint64_t switch_modulus, uint8_t *d_wire, size_t wire_stride,
+                                      void *stream, char *err, size_t err_cap) {
+    if (count && !d_wire) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    return respond_device(ctx, seat, ws, d_shard_ids, d_seeds, d_b, count, switch_modulus, nullptr, nullptr, stream,
+                          nullptr, err, err_cap, d_wire, wire_stride);
+}
+
+int inspire_respond_batch_wire(inspire_ctx *ctx, const inspire_seat *seat, const uint32_t *shard_ids,
+                               const uint8_t *seeds, const uint64_t *b, size_t count, uint64_t switch_modulus,
+                               uint8_t *wire_out, size_t wire_stride, char *err, size_t err_cap) {
+    if (count && (!seeds || !wire_out)) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    return respond_host(ctx, seat, shard_ids, seeds, nullptr, b, count, switch_modulus, nullptr, nullptr, err, err_cap,
+                        wire_out, wire_stride);
 }
 
 struct inspire_graph {
@@ -612,8 +655,9 @@ int inspire_workspace_status(inspire_workspace *ws, size_t count, int32_t *statu
 
 static int respond_host(inspire_ctx *ctx, const inspire_seat *seat, const uint32_t *shard_ids, const uint8_t *seeds,
                         const uint64_t *a_in, const uint64_t *b, size_t count, uint64_t switch_modulus,
-                        uint64_t *a_out, uint64_t *b_prefix_out, char *err, size_t err_cap) {
-    if (!ctx || (count && (!shard_ids || (!seeds && !a_in) || !b || !a_out || !b_prefix_out))) {
+                        uint64_t *a_out, uint64_t *b_prefix_out, char *err, size_t err_cap,
+                        uint8_t *wire_out, size_t wire_stride) {
+    if (!ctx || (count && (!shard_ids || (!seeds && !a_in) || !b || (!wire_out && (!a_out || !b_prefix_out))))) {
         set_err(err, err_cap, "null argument");
         return INSPIRE_ERR_INVALID;
     }
@@ -680,6 +724,20 @@ static int respond_host(inspire_ctx *ctx, const inspire_seat *seat, const uint32
     bd.b_out = ws->b_out.as<uint64_t>();
     bd.status = ws->status.as<int32_t>();
     bd.switch_modulus = switch_modulus;
+    const WireLayout wl = wire_layout((uint32_t)n, (uint32_t)g, switch_modulus ? switch_modulus : kQ);
+    const size_t dstride = (wl.total + 3) & ~(size_t)3;
+    if (wire_out) {
+        if (wire_stride < wl.total) {
+            set_err(err, err_cap, "response stride %zu is below the %u-byte response", wire_stride, wl.total);
+            return INSPIRE_ERR_INVALID;
+        }
+        if (ws->wire_cap < count * dstride) {
+            HIP_TRY(ws->wire.alloc(count * dstride));
+            ws->wire_cap = count * dstride;
+        }
+        bd.wire_out = ws->wire.as<uint8_t>();
+        bd.wire_stride = dstride;
+    }
     HIP_TRY(launch_respond(ctx->dev(), bd, 0));
     HIP_TRY(hipDeviceSynchronize());
     std::vector<int32_t> st(count);
@@ -692,8 +750,10 @@ static int respond_host(inspire_ctx *ctx, const inspire_seat *seat, const uint32
                     shard_ids[k], st[k] - 1);
             return INSPIRE_ERR_STORE;
         }
-    HIP_TRY(hipMemcpy(a_out, ws->a_out.p, count * n * 8, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(b_prefix_out, ws->b_out.p, count * g * 8, hipMemcpyDeviceToHost));
+    if (a_out) HIP_TRY(hipMemcpy(a_out, ws->a_out.p, count * n * 8, hipMemcpyDeviceToHost));
+    if (b_prefix_out) HIP_TRY(hipMemcpy(b_prefix_out, ws->b_out.p, count * g * 8, hipMemcpyDeviceToHost));
+    if (wire_out)
+        HIP_TRY(hipMemcpy2D(wire_out, wire_stride, ws->wire.p, dstride, wl.total, count, hipMemcpyDeviceToHost));
     return INSPIRE_OK;
 }
 

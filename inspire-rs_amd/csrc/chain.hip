@@ -219,6 +219,45 @@ __device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi
     }
 }
 
+// Byte p of `count` coefficients packed LSB-first at w bits each (pack_coefficients_tight,
+// poly.rs:166-225), read from an LDS array; zero past the last coefficient.
+__device__ inline uint32_t tight_byte(const uint64_t *L, uint32_t count, uint32_t w, uint32_t p) {
+    const uint32_t bit = 8 * p, i = bit / w, s = bit - i * w;
+    if (i >= count) return 0;
+    uint64_t v = L[lds_ix(i)] >> s;
+    if (w - s < 8 && i + 1 < count) v |= L[lds_ix(i + 1)] << (w - s);
+    return (uint32_t)(v & 0xff);
+}
+
+// Byte o of the response (layout in WireLayout); a from C, the b prefix from D.
+__device__ inline uint32_t wire_byte(const WireLayout &L, const uint64_t *C, const uint64_t *D, uint32_t o) {
+    auto le = [](uint64_t v, uint32_t k) -> uint32_t { return (uint32_t)(v >> (8 * k)) & 0xff; };
+    if (o < 4) return le(1, o);
+    o -= 4;
+    if (o < 8) return le(L.a_bytes, o);
+    o -= 8;
+    if (o < L.a_bytes) return tight_byte(C, L.n, L.w, o);
+    o -= L.a_bytes;
+    if (o < 40) {
+        const uint32_t f = o / 8;
+        const uint64_t v = f == 0 ? 1 : f <= 2 ? L.modulus : f == 3 ? L.n : 0;
+        return le(v, o % 8);
+    }
+    o -= 40;
+    if (o < 1) return 0;
+    o -= 1;
+    if (o < 8) return le(L.b_bytes, o);
+    o -= 8;
+    if (o < L.b_bytes) return tight_byte(D, L.gamma, L.w, o);
+    o -= L.b_bytes;
+    if (o < 4) return le(L.gamma, o);
+    o -= 4;
+    if (o < 8) return 0;
+    o -= 8;
+    if (o < 1) return 1;
+    return 0;  // PackingMode::Inspiring (u32 0) and the padding of the last word
+}
+
 template <int LOGN, int ROWS, bool EXACT>
 __global__ __launch_bounds__((1 << LOGN) / 4, 4) void k_chain(CtxDev c, BatchDev bt) {
     constexpr int N = 1 << LOGN;
@@ -411,13 +450,40 @@ __global__ __launch_bounds__((1 << LOGN) / 4, 4) void k_chain(CtxDev c, BatchDev
         __syncthreads();
     }
     const uint64_t qn = bt.switch_modulus;
-    for (int j = tid; j < N; j += T) {
-        uint64_t a = C[lds_ix(j)];
-        bt.a_out[(size_t)qi * N + j] = qn ? mod_switch_coeff(a, qn) : a;
-    }
-    for (int j = tid; j < gamma; j += T) {
-        uint64_t bv = addmod(bpoly[j], D[lds_ix(j)]);
-        bt.b_out[(size_t)qi * gamma + j] = qn ? mod_switch_coeff(bv, qn) : bv;
+    if (!bt.wire_out) {
+        for (int j = tid; j < N; j += T) {
+            uint64_t a = C[lds_ix(j)];
+            bt.a_out[(size_t)qi * N + j] = qn ? mod_switch_coeff(a, qn) : a;
+        }
+        for (int j = tid; j < gamma; j += T) {
+            uint64_t bv = addmod(bpoly[j], D[lds_ix(j)]);
+            bt.b_out[(size_t)qi * gamma + j] = qn ? mod_switch_coeff(bv, qn) : bv;
+        }
+    } else {
+        // final coefficients back in place (each thread its own slots), then the response
+        // bytes assembled from LDS, one 4-byte word per lane and pass
+        for (int j = tid; j < N; j += T) {
+            uint64_t a = C[lds_ix(j)];
+            a = qn ? mod_switch_coeff(a, qn) : a;
+            C[lds_ix(j)] = a;
+            if (bt.a_out) bt.a_out[(size_t)qi * N + j] = a;
+        }
+        for (int j = tid; j < gamma; j += T) {
+            uint64_t bv = addmod(bpoly[j], D[lds_ix(j)]);
+            bv = qn ? mod_switch_coeff(bv, qn) : bv;
+            D[lds_ix(j)] = bv;
+            if (bt.b_out) bt.b_out[(size_t)qi * gamma + j] = bv;
+        }
+        __syncthreads();
+        const WireLayout L = wire_layout(N, gamma, qn ? qn : kQ);
+        uint32_t *out = reinterpret_cast<uint32_t *>(bt.wire_out + (size_t)qi * bt.wire_stride);
+        const uint32_t words = (L.total + 3) / 4;
+        for (uint32_t wi = tid; wi < words; wi += T) {
+            uint32_t word = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) word |= wire_byte(L, C, D, 4 * wi + k) << (8 * k);
+            out[wi] = word;
+        }
     }
     if (tid == 0) bt.status[qi] = 0;
 }

@@ -1,5 +1,6 @@
 // Device-side descriptors shared by the kernels and the host runtime.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace inspire {
@@ -51,6 +52,33 @@ struct BatchDev {
     uint64_t *b_out;            // count x gamma
     int32_t *status;            // count: 0 ok, c+1 = column c failed its tag
     uint64_t switch_modulus;    // 0 = no modulus switch
+    // when non-null, each query's ServerResponse bincode bytes at wire_out + qi * wire_stride
+    // (4-byte aligned, stride a multiple of 4 holding the length rounded up to 4); a_out and
+    // b_out may then be null
+    uint8_t *wire_out = nullptr;
+    size_t wire_stride = 0;
 };
+
+// Layout of a packed InspiRING ServerResponse in bincode (respond.rs:111-176, poly.rs:166-225,
+// 286-310): u32 variant 1 | u64 A | a tight (A bytes) | u64 1 | u64 modulus | u64 modulus |
+// u64 n | u64 0 | u8 0 | u64 B | b prefix tight (B bytes) | u32 gamma | u64 0 | u8 1 | u32 0.
+struct WireLayout {
+    uint32_t w, a_bytes, b_bytes, total, n, gamma;
+    uint64_t modulus;
+};
+
+__host__ __device__ inline WireLayout wire_layout(uint32_t n, uint32_t gamma, uint64_t modulus) {
+    WireLayout L;
+    uint32_t w = 0;
+    while (w < 64 && ((modulus - 1) >> w) != 0) w++;
+    L.w = w;
+    L.a_bytes = (n * w + 7) / 8;
+    L.b_bytes = (gamma * w + 7) / 8;
+    L.total = L.a_bytes + L.b_bytes + 78;
+    L.n = n;
+    L.gamma = gamma;
+    L.modulus = modulus;
+    return L;
+}
 
 }  // namespace inspire

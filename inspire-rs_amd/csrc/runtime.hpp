@@ -116,7 +116,8 @@ struct inspire_workspace {
     size_t max_count = 0;
     DevBuf scan_part, status;
     // staging for the host-buffer entry point
-    DevBuf ids, seeds, a_in, b, a_out, b_out;
+    DevBuf ids, seeds, a_in, b, a_out, b_out, wire;
+    size_t wire_cap = 0;
 };
 
 struct inspire_ctx {

@@ -94,6 +94,10 @@ def lib():
         L.inspire_server_wait.argtypes = [vp, u64, vp, vp, ctypes.POINTER(ctypes.c_double), cp, sz]
         L.inspire_server_stats.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.inspire_workspace_status.argtypes = [vp, sz, vp, cp, sz]
+        L.inspire_response_wire_bytes.argtypes = [vp, u64]
+        L.inspire_response_wire_bytes.restype = sz
+        L.inspire_respond_batch_device_wire.argtypes = [vp, vp, vp, vp, vp, vp, sz, u64, vp, sz, vp, cp, sz]
+        L.inspire_respond_batch_wire.argtypes = [vp, vp, vp, vp, vp, sz, u64, vp, sz, cp, sz]
         L.inspire_respond_graph_create.argtypes = [vp, vp, vp, vp, vp, vp, sz, u64, vp, vp, pp, cp, sz]
         L.inspire_graph_launch.argtypes = [vp, vp, cp, sz]
         L.inspire_graph_destroy.argtypes = [vp]
@@ -285,6 +289,29 @@ class Context:
         finally:
             seat.close()
         return a[0], bp[0]
+
+    def response_bytes(self, switch_modulus=0):
+        """Length of one ServerResponse in bincode at this context's shape."""
+        return lib().inspire_response_wire_bytes(self._h, switch_modulus)
+
+    def respond_seeded_batch_wire(self, seat, shard_ids, seeds, bs, switch_modulus=0):
+        """respond_seeded_batch with each response's to_binary bytes packed on the device."""
+        ids = np.ascontiguousarray(shard_ids, dtype=np.uint32)
+        count = len(ids)
+        sd = np.ascontiguousarray(np.frombuffer(b"".join(bytes(s) for s in seeds), dtype=np.uint8))
+        b = np.ascontiguousarray(bs, dtype=np.uint64).reshape(count, self.n)
+        size = self.response_bytes(switch_modulus)
+        out = np.zeros((count, size), dtype=np.uint8)
+        _call(lib().inspire_respond_batch_wire, self._h, seat._h, _ptr(ids), _ptr(sd), _ptr(b), count, switch_modulus,
+              _ptr(out), size)
+        return [out[k].tobytes() for k in range(count)]
+
+    def respond_batch_device_wire(self, seat, ws, d_ids, d_seeds, d_b, count, d_wire, wire_stride, stream=0,
+                                  switch_modulus=0):
+        """Device-pointer batch writing response bytes at d_wire + k * wire_stride."""
+        _call(lib().inspire_respond_batch_device_wire, self._h, seat._h, ws._h, ctypes.c_void_p(d_ids),
+              ctypes.c_void_p(d_seeds), ctypes.c_void_p(d_b), count, switch_modulus, ctypes.c_void_p(d_wire),
+              wire_stride, ctypes.c_void_p(stream))
 
     def respond_batch_device(self, seat, ws, d_ids, d_seeds, d_b, count, d_a_out, d_b_out, stream=0,
                              switch_modulus=0):

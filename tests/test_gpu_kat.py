@@ -87,3 +87,17 @@ def test_served_by_handle_serves_the_recorded_bytes(gpu):
     for (index, (length, fnv)), (a, b) in zip(recs, got):
         wire = gpu.encode_response(a, b)
         assert (len(wire), oracle.fnv1a(wire)) == (length, fnv), f"index {index}"
+
+
+@pytest.mark.parametrize("cell", CELLS, ids=lambda c: f"{c[0]}-e{c[1]}-{c[2]:#x}")
+def test_device_packed_response_bytes_are_the_recorded_bytes(gpu, cell):
+    """The packing kernel's wire epilogue writes ServerResponse::to_binary itself."""
+    c, recs, queries = _cell_inputs(cell)
+    print(f"kat cell {cell}: device wire bytes", flush=True)
+    with _context(gpu, c, precompute=True) as ctx:
+        seat = ctx.seat(c.y_body)
+        wires = ctx.respond_seeded_batch_wire(seat, [q[0] for q in queries], [q[1] for q in queries],
+                                              np.stack([q[2] for q in queries]))
+        seat.close()
+    for (index, (length, fnv)), wire in zip(recs, wires):
+        assert (len(wire), oracle.fnv1a(wire)) == (length, fnv), f"{cell} index {index}"
