@@ -342,7 +342,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="time stream launches instead of hipGraph replays")
     ap.add_argument("--no-autotune", action="store_true", help="keep --queries-per-cu instead of autotuning")
     ap.add_argument("--served-seconds", type=float, default=4.0)
-    ap.add_argument("--queries-per-cu", type=int, default=2, help="launch shaping of the packing kernel (1 or 2)")
+    ap.add_argument("--queries-per-cu", type=int, default=2, help="launch shaping of the packing kernel (1, 2 or 3)")
     ap.add_argument("--shards", type=int, default=128)
     ap.add_argument("--plain-shards", type=int, default=32)
     ap.add_argument("--plain-queries", type=int, default=16)

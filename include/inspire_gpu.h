@@ -169,10 +169,10 @@ void inspire_graph_destroy(inspire_graph *graph);
 
 /* Launch-shape autotuner (no reference counterpart; results are unchanged): times the
  * respond pipeline on synthetic queries of the real shape (count queries over the
- * context's stored shards) at 1 and 2 queries per CU, keeps the faster shape in the
- * context, and reports both times in kernel_ms and the choice in *chosen. */
+ * context's stored shards) at 1, 2 and 3 queries per CU, keeps the fastest shape in the
+ * context, and reports the three times in kernel_ms and the choice in *chosen. */
 int inspire_ctx_autotune(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws, size_t count,
-                         float kernel_ms[2], uint32_t *chosen, char *err, size_t err_cap);
+                         float kernel_ms[3], uint32_t *chosen, char *err, size_t err_cap);
 
 /* Continuous-batching server over one context (the serving side of
  * respond_seeded_batch with registered keys, src/pir/respond.rs:1189, and of

@@ -213,8 +213,8 @@ void inspire_ctx_destroy(inspire_ctx *ctx) {
 uint32_t inspire_ctx_key_rows(const inspire_ctx *ctx) { return ctx ? (uint32_t)ctx->rows : 0; }
 
 int inspire_ctx_set_queries_per_cu(inspire_ctx *ctx, uint32_t queries_per_cu, char *err, size_t err_cap) {
-    if (!ctx || queries_per_cu < 1 || queries_per_cu > 2) {
-        set_err(err, err_cap, "queries per CU must be 1 or 2");
+    if (!ctx || queries_per_cu < 1 || queries_per_cu > 3) {
+        set_err(err, err_cap, "queries per CU must be 1, 2 or 3");
         return INSPIRE_ERR_INVALID;
     }
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -590,7 +590,7 @@ int inspire_graph_launch(inspire_graph *g, void *stream, char *err, size_t err_c
 void inspire_graph_destroy(inspire_graph *g) { delete g; }
 
 int inspire_ctx_autotune(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws, size_t count,
-                         float kernel_ms[2], uint32_t *chosen, char *err, size_t err_cap) {
+                         float kernel_ms[3], uint32_t *chosen, char *err, size_t err_cap) {
     if (!ctx || !ws || ws->ctx != ctx || !kernel_ms || !chosen) {
         set_err(err, err_cap, "null argument or workspace of another context");
         return INSPIRE_ERR_INVALID;
@@ -630,7 +630,7 @@ int inspire_ctx_autotune(inspire_ctx *ctx, const inspire_seat *seat, inspire_wor
     const int prev = ctx->queries_per_cu;
     float best = 0;
     uint32_t pick = (uint32_t)prev;
-    for (int qpc = 1; qpc <= 2; qpc++) {
+    for (int qpc = 1; qpc <= 3; qpc++) {
         ctx->queries_per_cu = qpc;
         float t[2] = {0, 0}, sum = 0;
         for (int rep = 0; rep < 3; rep++) {  // one warm-up, two timed

@@ -258,14 +258,16 @@ __device__ inline uint32_t wire_byte(const WireLayout &L, const uint64_t *C, con
     return 0;  // PackingMode::Inspiring (u32 0) and the padding of the last word
 }
 
-template <int LOGN, int ROWS, bool EXACT>
-__global__ __launch_bounds__((1 << LOGN) / 4, 4) void k_chain(CtxDev c, BatchDev bt) {
+// OCC is the number of waves per SIMD the register allocation targets: 4 for one or two
+// queries per CU, 6 for three (three workgroups' LDS fit a CU; registers then cap at 80).
+template <int LOGN, int ROWS, bool EXACT, int OCC>
+__global__ __launch_bounds__((1 << LOGN) / 4) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
+void k_chain(CtxDev c, BatchDev bt) {
     constexpr int N = 1 << LOGN;
     constexpr int T = N / 4;
     __shared__ uint64_t A[LdsSize<LOGN>::value];  // NTT(a)
     __shared__ uint64_t C[LdsSize<LOGN>::value];  // transform buffer / digit 0
     __shared__ uint64_t D[LdsSize<LOGN>::value];  // digit 1
-    __shared__ uint64_t bpoly[1024];
     __shared__ int bad_column;
     __shared__ int rejected;
 
@@ -277,6 +279,9 @@ __global__ __launch_bounds__((1 << LOGN) / 4, 4) void k_chain(CtxDev c, BatchDev
     // --- finish the scan: b_poly and the column tags, before any packing work
     if (tid == 0) { bad_column = 0x7fffffff; rejected = 0; }
     __syncthreads();
+    // b_poly[j] is parked in the first word of the query's own scan partials for column j,
+    // which only this thread reads, until the epilogue (keeps it out of LDS)
+    uint64_t *const bpoly = bt.scan_part + (size_t)qi * bt.ks * gamma * 4;
     for (int j = tid; j < gamma; j += T) {
         uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
         for (int s = 0; s < bt.ks; s++) {
@@ -284,7 +289,7 @@ __global__ __launch_bounds__((1 << LOGN) / 4, 4) void k_chain(CtxDev c, BatchDev
             s0 += p[0]; s1 += p[1]; s2 += p[2]; s3 += p[3];
         }
         uint64_t lo = s0 + (s1 << 32), hi = (s1 >> 32) + (lo < s0);
-        bpoly[j] = reduce120(hi, lo);
+        bpoly[4 * j] = reduce120(hi, lo);
         uint64_t tlo = s2 + (s3 << 32), thi = (s3 >> 32) + (tlo < s2);
         if (mod_p61(thi, tlo) != sh.tags[j]) atomicMin(&bad_column, j);
     }
@@ -456,7 +461,7 @@ __global__ __launch_bounds__((1 << LOGN) / 4, 4) void k_chain(CtxDev c, BatchDev
             bt.a_out[(size_t)qi * N + j] = qn ? mod_switch_coeff(a, qn) : a;
         }
         for (int j = tid; j < gamma; j += T) {
-            uint64_t bv = addmod(bpoly[j], D[lds_ix(j)]);
+            uint64_t bv = addmod(bpoly[4 * j], D[lds_ix(j)]);
             bt.b_out[(size_t)qi * gamma + j] = qn ? mod_switch_coeff(bv, qn) : bv;
         }
     } else {
@@ -469,7 +474,7 @@ __global__ __launch_bounds__((1 << LOGN) / 4, 4) void k_chain(CtxDev c, BatchDev
             if (bt.a_out) bt.a_out[(size_t)qi * N + j] = a;
         }
         for (int j = tid; j < gamma; j += T) {
-            uint64_t bv = addmod(bpoly[j], D[lds_ix(j)]);
+            uint64_t bv = addmod(bpoly[4 * j], D[lds_ix(j)]);
             bv = qn ? mod_switch_coeff(bv, qn) : bv;
             D[lds_ix(j)] = bv;
             if (bt.b_out) bt.b_out[(size_t)qi * gamma + j] = bv;
@@ -488,21 +493,26 @@ __global__ __launch_bounds__((1 << LOGN) / 4, 4) void k_chain(CtxDev c, BatchDev
     if (tid == 0) bt.status[qi] = 0;
 }
 
-// One query per CU unless asked otherwise: a workgroup then reserves more than half
-// of the CU's 160 KiB LDS (dynamic padding, never touched), so the dispatcher spreads
-// a launch over all CUs instead of pairing queries on some and idling others.
+// At most queries_per_cu workgroups per CU: a workgroup reserves more than
+// 1/(queries_per_cu + 1) of the CU's 160 KiB LDS (dynamic padding, never touched), so the
+// dispatcher spreads a launch over all CUs instead of stacking queries on some and
+// idling others. Three per CU use the kernel compiled for 6 waves per SIMD.
 constexpr size_t kLdsPerCu = 160 * 1024;
 
 template <int LOGN>
 static hipError_t launch_chain_t(const CtxDev &c, const BatchDev &b, hipStream_t s) {
     constexpr int T = (1 << LOGN) / 4;
-    auto kern = c.rows == 3 ? k_chain<LOGN, 3, true> : k_chain<LOGN, 2, false>;
+    const bool occ6 = c.queries_per_cu >= 3;
+    auto kern = c.rows == 3 ? (occ6 ? k_chain<LOGN, 3, true, 6> : k_chain<LOGN, 3, true, 4>)
+                            : (occ6 ? k_chain<LOGN, 2, false, 6> : k_chain<LOGN, 2, false, 4>);
     hipFuncAttributes attr;
     hipError_t e = hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(kern));
     if (e != hipSuccess) return e;
     size_t dyn = 0;
-    if (c.queries_per_cu <= 1 && attr.sharedSizeBytes <= kLdsPerCu / 2)
-        dyn = kLdsPerCu / 2 + 256 - attr.sharedSizeBytes;
+    if (c.queries_per_cu <= 2) {
+        const size_t share = kLdsPerCu / (size_t)(c.queries_per_cu + 1) + 256;
+        if (attr.sharedSizeBytes < share) dyn = share - attr.sharedSizeBytes;
+    }
     hipLaunchKernelGGL(kern, dim3(b.count), dim3(T), dyn, s, c, b);
     return hipGetLastError();
 }

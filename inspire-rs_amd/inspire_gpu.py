@@ -342,11 +342,12 @@ class RespondGraph:
 
 
 def autotune(ctx, seat, ws, count):
-    """inspire_ctx_autotune: times 1 and 2 queries per CU at `count` queries, keeps the faster."""
-    ms = (ctypes.c_float * 2)()
+    """inspire_ctx_autotune: times 1, 2 and 3 queries per CU at `count` queries, keeps the fastest."""
+    ms = (ctypes.c_float * 3)()
     chosen = ctypes.c_uint32()
     _call(lib().inspire_ctx_autotune, ctx._h, seat._h, ws._h, count, ms, ctypes.byref(chosen))
-    return {"queries_per_cu": chosen.value, "ms_qpc1": round(ms[0], 4), "ms_qpc2": round(ms[1], 4)}
+    return {"queries_per_cu": chosen.value, "ms_qpc1": round(ms[0], 4), "ms_qpc2": round(ms[1], 4),
+            "ms_qpc3": round(ms[2], 4)}
 
 
 class Server:
