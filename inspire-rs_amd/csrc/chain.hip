@@ -13,9 +13,10 @@
 // Arithmetic: every product on the chain has one operand fixed ahead of the query
 // (twiddles, the store's S_i, the mask rows w, the key rows y, the last-digit
 // constants), so all of them are Shoup products a*w - floor(a*w'/2^64)*q with the
-// precomputed companion w' = floor(w*2^64/q). Residues stay lazy in [0, 2q) and are
-// made canonical only where the reference's bytes depend on it (the digits and the
-// outputs). The inverse transform carries no n^-1: S_i, w and y hold it already.
+// precomputed companion w' = floor(w*2^64/q), with a three-multiply quotient estimate.
+// Residues stay lazy, in [0, 4q) through the inverse transforms and the state and in
+// [0, 8q) through the forward transforms, and are made canonical only where the
+// reference's bytes depend on it (the digits and the outputs). The inverse transform carries no n^-1: S_i, w and y hold it already.
 //
 // Layout: thread t owns slots 4t..4t+3. The first inverse pass (bits 0-1) and the
 // last forward pass (bits 1-0) work on exactly those slots, so the recursion state
@@ -32,29 +33,31 @@
 
 namespace inspire {
 
-__device__ __forceinline__ uint64_t shoup(uint64_t a, uint64_t w, uint64_t wp) {
-    uint64_t hi;
-    (void)mul_wide(a, wp, hi);
-    return a * w - hi * kQ;  // in [0, 2q) for any a < 2^64
-}
+// Every product here is shoup_lazy (modq.hpp): result in [0, 4q) for any a < 2^64.
+__device__ __forceinline__ uint64_t shoup(uint64_t a, uint64_t w, uint64_t wp) { return shoup_lazy(a, w, wp); }
 __device__ __forceinline__ uint64_t shoup(uint64_t a, const Pair &p) { return shoup(a, p.w, p.wp); }
 
+constexpr uint64_t kFourQ = 4 * kQ;
+constexpr uint64_t kEightQ = 8 * kQ;  // < 2^63; 16q < 2^64 bounds the pointwise sums
+__device__ __forceinline__ uint64_t csub4q(uint64_t x) { return x >= kFourQ ? x - kFourQ : x; }
 __device__ __forceinline__ uint64_t csub2q(uint64_t x) { return x >= kTwoQ ? x - kTwoQ : x; }
 __device__ __forceinline__ uint64_t csubq(uint64_t x) { return x >= kQ ? x - kQ : x; }
+// [0, 4q) -> canonical
+__device__ __forceinline__ uint64_t canon4(uint64_t x) { return csubq(csub2q(x)); }
 
-// Gentleman-Sande butterfly on [0, 2q) residues.
+// Gentleman-Sande butterfly: inputs and outputs in [0, 4q).
 __device__ __forceinline__ void gs(uint64_t &x, uint64_t &y, const Pair &w) {
-    uint64_t s = csub2q(x + y);
-    uint64_t d = x + kTwoQ - y;
+    uint64_t s = csub4q(x + y);
+    uint64_t d = x + kFourQ - y;  // (0, 8q)
     x = s;
     y = shoup(d, w);
 }
-// Cooley-Tukey butterfly (Harvey): inputs and outputs in [0, 4q).
+// Cooley-Tukey butterfly (Harvey's lazy form): inputs and outputs in [0, 8q).
 __device__ __forceinline__ void ct(uint64_t &x, uint64_t &y, const Pair &w) {
-    uint64_t t = shoup(y, w);
-    uint64_t u = csub2q(x);
+    uint64_t t = shoup(y, w);  // [0, 4q)
+    uint64_t u = csub4q(x);    // [0, 4q)
     x = u + t;
-    y = u + kTwoQ - t;
+    y = u + kFourQ - t;
 }
 
 // Radix-4 inverse pass over bits (b, b+1) on a group's registers v[k] = x[base + (k << b)].
@@ -178,8 +181,8 @@ __device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi
             uint64_t x = C[lds_ix(e)], y = C[lds_ix(e + N / 2)];
             gs(x, y, twi[1]);
             uint64_t a0, a1, b0, b1;
-            digits<EXACT>(csubq(x), a0, a1);
-            digits<EXACT>(csubq(y), b0, b1);
+            digits<EXACT>(canon4(x), a0, a1);
+            digits<EXACT>(canon4(y), b0, b1);
             ct(a0, b0, twf[1]);
             ct(a1, b1, twf[1]);
             C[lds_ix(e)] = a0;
@@ -202,7 +205,7 @@ __device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi
         for (int k = 0; k < 4; k++) v[k] = C[lds_ix(base + (k << b))];
         gs4<LOGN>(v, b, hi, twi);
 #pragma unroll
-        for (int k = 0; k < 4; k++) digits<EXACT>(csubq(v[k]), u[k], w[k]);
+        for (int k = 0; k < 4; k++) digits<EXACT>(canon4(v[k]), u[k], w[k]);
         ct4<LOGN>(u, b, hi, twf);
         ct4<LOGN>(w, b, hi, twf);
 #pragma unroll
@@ -406,11 +409,11 @@ void k_chain(CtxDev c, BatchDev bt) {
                 r += shoup(d[rr], c.wprime[(size_t)rr * N + src]);
                 o += shoup(d[rr], Y[(size_t)rr * N + src]);
             }
-            // r < (1 + ROWS) * 2q <= 8q, o < (1 + ROWS) * 2q: back to [0, 2q)
-            r = r >= 4 * kQ ? r - 4 * kQ : r;
-            o = o >= 4 * kQ ? o - 4 * kQ : o;
-            X[k] = csub2q(r);
-            acc[k] = csub2q(o);
+            // r, o < (1 + ROWS) * 4q <= 16q < 2^64: back to [0, 4q)
+            r = r >= kEightQ ? r - kEightQ : r;
+            o = o >= kEightQ ? o - kEightQ : o;
+            X[k] = csub4q(r);
+            acc[k] = csub4q(o);
         }
     }
 
@@ -437,19 +440,19 @@ void k_chain(CtxDev c, BatchDev bt) {
                 const int e = tid + r * (N / 4);
                 uint64_t x = C[lds_ix(e)], y = C[lds_ix(e + N / 2)];
                 gs(x, y, twi[1]);
-                C[lds_ix(e)] = csubq(x);
-                C[lds_ix(e + N / 2)] = csubq(y);
+                C[lds_ix(e)] = canon4(x);
+                C[lds_ix(e + N / 2)] = canon4(y);
                 x = D[lds_ix(e)], y = D[lds_ix(e + N / 2)];
                 gs(x, y, twi[1]);
-                D[lds_ix(e)] = csubq(x);
-                D[lds_ix(e + N / 2)] = csubq(y);
+                D[lds_ix(e)] = canon4(x);
+                D[lds_ix(e + N / 2)] = canon4(y);
             }
         } else {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int e = tid + r * T;
-                C[lds_ix(e)] = csubq(C[lds_ix(e)]);
-                D[lds_ix(e)] = csubq(D[lds_ix(e)]);
+                C[lds_ix(e)] = canon4(C[lds_ix(e)]);
+                D[lds_ix(e)] = canon4(D[lds_ix(e)]);
             }
         }
         __syncthreads();

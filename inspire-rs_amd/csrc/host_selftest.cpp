@@ -30,5 +30,6 @@ uint64_t st_shoup(uint64_t a, uint64_t w) {
     (void)mul_wide(a, wp, hi);
     return a * w - hi * kQ;
 }
+uint64_t st_shoup_lazy(uint64_t a, uint64_t w) { return shoup_lazy(a, w, shoup_companion(w)); }
 int st_uniform_q_draw(uint64_t v, uint64_t *out) { return uniform_q_draw(v, *out) ? 1 : 0; }
 }

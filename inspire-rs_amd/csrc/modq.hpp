@@ -135,6 +135,19 @@ __host__ __device__ __forceinline__ uint64_t shoup_companion(uint64_t w) {
     return e;
 }
 
+// Shoup product a*w mod q with the companion w' = floor(w * 2^64 / q), for any a < 2^64.
+// The quotient estimate keeps a1*w'1 and the high halves of the two middle partial
+// products, leaving out a0*w'0 and the middle carries: it is at most 2 short of
+// floor(a*w'/2^64) (the dropped part is below 3 * 2^32 in units of 2^32), and that
+// floor is itself within 1 of floor(a*w/q), so the result is in [0, 4q). Three 32-bit
+// multiplies for the quotient instead of four plus their carry moves.
+__host__ __device__ __forceinline__ uint64_t shoup_lazy(uint64_t a, uint64_t w, uint64_t wp) {
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+    const uint32_t p0 = (uint32_t)wp, p1 = (uint32_t)(wp >> 32);
+    const uint64_t est = mul32w(a1, p1) + (mul32w(a1, p0) >> 32) + (mul32w(a0, p1) >> 32);
+    return a * w - est * kQ;
+}
+
 __host__ __device__ __forceinline__ uint32_t bitrev(uint32_t x, int bits) {
 #ifdef __HIP_DEVICE_COMPILE__
     return __builtin_bitreverse32(x) >> (32 - bits);

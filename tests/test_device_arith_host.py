@@ -40,6 +40,8 @@ def st():
     L.st_shoup.argtypes = [u64, u64]
     L.st_shoup.restype = u64
     L.st_uniform_q_draw.argtypes = [u64, ctypes.POINTER(u64)]
+    L.st_shoup_lazy.argtypes = [u64, u64]
+    L.st_shoup_lazy.restype = u64
     return L
 
 
@@ -143,3 +145,30 @@ def test_shoup_companion_and_product_range(st):
         w = int(rng.integers(0, Q, dtype=np.uint64))
         r = st.st_shoup(a, w)
         assert r < 2 * Q and r % Q == a * w % Q
+
+
+def test_lazy_shoup_range_and_residue(st):
+    """The packing chain's product: a*w (mod q) in [0, 4q) for every a < 2^64, including the
+    operands that maximise the dropped low partial products and middle carries."""
+    rng = np.random.default_rng(7)
+    M32 = (1 << 32) - 1
+    worst_a = [2**64 - 1, (M32 << 32) | M32, (1 << 63) - 1, 8 * Q - 1, 16 * Q - 1, 4 * Q - 1, M32, M32 << 32]
+    ws = [Q - 1, Q - 2, 1, 0, Q // 2, (1 << 59) + 7] + [int(x) for x in rng.integers(0, Q, 200, dtype=np.uint64)]
+    seen_above_2q = False
+    for a in worst_a + [int(x) for x in rng.integers(0, 2**64 - 1, 400, dtype=np.uint64)]:
+        for w in ws[:60]:
+            r = st.st_shoup_lazy(a, w)
+            assert r < 4 * Q and r % Q == a * w % Q, (a, w, r)
+            seen_above_2q |= r >= 2 * Q
+    # search for operands whose estimate is 2 short: the bound is reached, not loose
+    short2 = 0
+    for _ in range(20000):
+        a = int(rng.integers(0, 2**64 - 1, dtype=np.uint64))
+        w = int(rng.integers(0, Q, dtype=np.uint64))
+        wp = (w << 64) // Q
+        a0, a1, p0, p1 = a & M32, a >> 32, wp & M32, wp >> 32
+        est = a1 * p1 + ((a1 * p0) >> 32) + ((a0 * p1) >> 32)
+        short2 += (a * wp >> 64) - est == 2
+        r = st.st_shoup_lazy(a, w)
+        assert r < 4 * Q and r % Q == a * w % Q
+    assert short2 > 0 and seen_above_2q
