@@ -504,7 +504,7 @@ def main():
     pd_b = torch.from_numpy(pbs.view(np.int64)).to(dev)
     pa = torch.empty((1, N), dtype=torch.int64, device=dev)
     pb = torch.empty((1, GAMMA), dtype=torch.int64, device=dev)
-    for i in range(2):
+    for i in range(min(2, P)):
         ctxp.respond_batch_device(seatp, wsp, pd_ids[i].data_ptr(), pd_seeds[i].data_ptr(), pd_b[i].data_ptr(), 1,
                                   pa.data_ptr(), pb.data_ptr(), stream=stream, switch_modulus=Q36)
     torch.cuda.synchronize()
