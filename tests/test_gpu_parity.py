@@ -36,7 +36,8 @@ def test_d256_reference_fixture_byte_identity(gpu):
             fx["w_seed"], fx["a"], fx["b"], fx["columns"], fx["y_body"]), g)
 
 
-@pytest.mark.parametrize("gamma,approx", [(16, False), (256, False), (16, True), (256, True), (512, False)])
+@pytest.mark.parametrize("gamma,approx", [(16, False), (256, False), (16, True), (256, True), (512, False),
+                                          (1, False), (2, True), (1024, False)])
 def test_d2048_seeded_matches_oracle(gpu, gamma, approx):
     n = 2048
     rows = 2 if approx else 3
