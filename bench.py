@@ -467,6 +467,29 @@ def main():
     qps = L * launches / wall
     eager_qps = L * launches / eager_wall
 
+    # the same queries answered one 64-query batch per launch (no merging across batches)
+    single = None
+    if M > 1:
+        log("bench: one batch per launch")
+        nb = min(launches * M, 32)
+        t_starts = [torch.cuda.Event(enable_timing=True) for _ in range(nb)]
+        t_ends = [torch.cuda.Event(enable_timing=True) for _ in range(nb)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(nb):
+            sl, off = args.warmup + k // M, (k % M) * B
+            t_starts[k].record()
+            ctx.respond_batch_device(seat, ws, d_ids[sl, off:].data_ptr(), d_seeds[sl, off:].data_ptr(),
+                                     d_b[sl, off:].data_ptr(), B, d_a_out.data_ptr(), d_b_out.data_ptr(),
+                                     stream=stream, switch_modulus=Q36)
+            t_ends[k].record()
+        torch.cuda.synchronize()
+        sw = time.perf_counter() - t0
+        slat = np.array([s_.elapsed_time(e) for s_, e in zip(t_starts, t_ends)])
+        single = {"value": round(B * nb / sw, 2), "unit": "queries/s", "batches": nb,
+                  "latency_ms": {"p50": round(float(np.percentile(slat, 50)), 3),
+                                 "p99": round(float(np.percentile(slat, 99)), 3)}}
+
     def step(i):
         return g.respond_batch_device_timed(ctx, seat, ws, d_ids[i].data_ptr(), d_seeds[i].data_ptr(),
                                             d_b[i].data_ptr(), L, d_a_out.data_ptr(), d_b_out.data_ptr(),
@@ -571,6 +594,7 @@ def main():
         "db_gbs": round(qps * N * ENTRY / 1e9, 3),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3)},
+        "one_batch_per_launch": single,
         "stream_launches": {"value": round(eager_qps, 2), "unit": "queries/s",
                             "latency_ms": {"p50": round(float(np.percentile(eager_lat, 50)), 3),
                                            "p99": round(float(np.percentile(eager_lat, 99)), 3)}},

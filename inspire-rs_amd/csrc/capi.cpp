@@ -399,7 +399,7 @@ static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_wo
                           const uint32_t *d_shard_ids, const uint8_t *d_seeds, const uint64_t *d_b, size_t count,
                           uint64_t switch_modulus, uint64_t *d_a_out, uint64_t *d_b_prefix_out, void *stream,
                           float *timing, char *err, size_t err_cap, uint8_t *d_wire = nullptr,
-                          size_t wire_stride = 0) {
+                          size_t wire_stride = 0, int queries_per_cu = 0) {
     if (!ctx || !ws || ws->ctx != ctx) { set_err(err, err_cap, "workspace belongs to another context"); return INSPIRE_ERR_INVALID; }
     int rc = check_batch(ctx, seat, nullptr, count, err, err_cap);
     if (rc) return rc;
@@ -412,11 +412,14 @@ static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_wo
                 (unsigned long long)switch_modulus);
         return INSPIRE_ERR_INVALID;
     }
+    CtxDev dev;
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
         rc = sync_table(ctx, err, err_cap);
         if (rc) return rc;
+        dev = ctx->dev();
     }
+    if (queries_per_cu) dev.queries_per_cu = queries_per_cu;  // a trial shape (autotune)
     BatchDev b;
     b.count = (int)count;
     b.shard_ids = d_shard_ids;
@@ -447,7 +450,7 @@ static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_wo
         return INSPIRE_ERR_INVALID;
     }
     if (!timing) {
-        HIP_TRY(launch_respond(ctx->dev(), b, static_cast<hipStream_t>(stream)));
+        HIP_TRY(launch_respond(dev, b, static_cast<hipStream_t>(stream)));
         return INSPIRE_OK;
     }
     // per-kernel durations from events on the launch stream
@@ -455,9 +458,9 @@ static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_wo
     hipEvent_t ev[3];
     for (int i = 0; i < 3; i++) HIP_TRY(hipEventCreate(&ev[i]));
     hipError_t e = hipEventRecord(ev[0], st);
-    if (e == hipSuccess) e = launch_scan_only(ctx->dev(), b, st);
+    if (e == hipSuccess) e = launch_scan_only(dev, b, st);
     if (e == hipSuccess) e = hipEventRecord(ev[1], st);
-    if (e == hipSuccess) e = launch_chain_only(ctx->dev(), b, st);
+    if (e == hipSuccess) e = launch_chain_only(dev, b, st);
     if (e == hipSuccess) e = hipEventRecord(ev[2], st);
     if (e == hipSuccess) e = hipEventSynchronize(ev[2]);
     if (e == hipSuccess) e = hipEventElapsedTime(&timing[0], ev[0], ev[1]);
@@ -627,22 +630,26 @@ int inspire_ctx_autotune(inspire_ctx *ctx, const inspire_seat *seat, inspire_wor
     HIP_TRY(hipMemcpy(d_ids.p, ids.data(), count * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d_seeds.p, seeds.data(), count * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d_b.p, b.data(), count * n * 8, hipMemcpyHostToDevice));
-    const int prev = ctx->queries_per_cu;
+    // each shape is tried on its own launches; the context's shape changes only at the end,
+    // so launches from other threads meanwhile keep the shape they had
     float best = 0;
-    uint32_t pick = (uint32_t)prev;
+    uint32_t pick = 1;
     for (int qpc = 1; qpc <= 3; qpc++) {
-        ctx->queries_per_cu = qpc;
         float t[2] = {0, 0}, sum = 0;
         for (int rep = 0; rep < 3; rep++) {  // one warm-up, two timed
             int rc = respond_device(ctx, seat, ws, d_ids.as<uint32_t>(), d_seeds.as<uint8_t>(), d_b.as<uint64_t>(),
-                                    count, 0, d_a.as<uint64_t>(), d_bp.as<uint64_t>(), nullptr, t, err, err_cap);
-            if (rc) { ctx->queries_per_cu = prev; return rc; }
+                                    count, 0, d_a.as<uint64_t>(), d_bp.as<uint64_t>(), nullptr, t, err, err_cap,
+                                    nullptr, 0, qpc);
+            if (rc) return rc;
             if (rep) sum += t[0] + t[1];
         }
         kernel_ms[qpc - 1] = sum / 2;
         if (qpc == 1 || sum / 2 < best) { best = sum / 2; pick = (uint32_t)qpc; }
     }
-    ctx->queries_per_cu = (int)pick;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->queries_per_cu = (int)pick;
+    }
     *chosen = pick;
     return INSPIRE_OK;
 }

@@ -129,10 +129,12 @@ int inspire_server::launch(Slot &s, char *err, size_t err_cap) {
         q += c;
     }
     s.count = q;
+    CtxDev dev;
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
         int rc = sync_table(ctx, err, err_cap);
         if (rc) return rc;
+        dev = ctx->dev();  // the shard table and launch shape as of this launch
     }
     HIP_TRY(hipMemcpyAsync(s.d_ids.p, s.h_ids, q * 4, hipMemcpyHostToDevice, stream));
     HIP_TRY(hipMemcpyAsync(s.d_seeds.p, s.h_seeds, q * 32, hipMemcpyHostToDevice, stream));
@@ -152,7 +154,7 @@ int inspire_server::launch(Slot &s, char *err, size_t err_cap) {
     bd.b_out = s.d_bp.as<uint64_t>();
     bd.status = s.status.as<int32_t>();
     bd.switch_modulus = s.reqs.front().switch_modulus;
-    HIP_TRY(launch_respond(ctx->dev(), bd, stream));
+    HIP_TRY(launch_respond(dev, bd, stream));
     HIP_TRY(hipMemcpyAsync(s.h_a, s.d_a.p, q * n * 8, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipMemcpyAsync(s.h_bp, s.d_bp.p, q * g * 8, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipMemcpyAsync(s.h_status, s.status.p, q * 4, hipMemcpyDeviceToHost, stream));
