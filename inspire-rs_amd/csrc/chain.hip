@@ -138,7 +138,7 @@ __device__ __forceinline__ void lds_fwd_pass2(uint64_t *x, uint64_t *y, int b, c
 // including the passes that the caller fuses. Leaves x holding the values after the
 // inverse passes over bits [0, stop), with a barrier at the end.
 template <int LOGN>
-__device__ __forceinline__ void inverse_to(uint64_t s[4], uint64_t *x, int stop, const Pair *tw) {
+__device__ __forceinline__ void inverse_to(uint64_t s[4], uint64_t *x, int stop, const Pair *tw, const Pair *tw_l) {
     uint64_t v[4] = {s[0], s[1], s[2], s[3]};
     gs4<LOGN>(v, 0, threadIdx.x, tw);
     const int base = 4 * threadIdx.x;
@@ -146,7 +146,7 @@ __device__ __forceinline__ void inverse_to(uint64_t s[4], uint64_t *x, int stop,
     for (int k = 0; k < 4; k++) x[lds_ix(base + k)] = v[k];
     step_sync(stop > 2);  // wave-local when the pass over bits 2-3 follows
     for (int b = 2; b < stop; b += 2) {
-        lds_inv_pass<LOGN>(x, b, tw);
+        lds_inv_pass<LOGN>(x, b, tw_l);
         // the next pass is over bits b+2, b+3, or the top of the transform
         step_sync(b + 4 <= 8 && b + 2 < stop);
     }
@@ -278,6 +278,14 @@ void k_chain(CtxDev c, BatchDev bt) {
     const ShardDev sh = c.shards[bt.shard_ids[qi]];
     const Pair *twf = c.twf, *twi = c.twi;
     const Pair *Y = bt.yprimes ? bt.yprimes[qi] : bt.yprime;
+    // Twiddles of every pass above the own-slot one (ladder entries below n/4: 16 KiB at
+    // n = 2048) live in LDS for the whole kernel; the own-slot passes read theirs from L2.
+    __shared__ Pair TWL[2][(1 << LOGN) / 4];
+    for (int m = tid; m < N / 4; m += T) {
+        TWL[0][m] = twf[m];
+        TWL[1][m] = twi[m];
+    }
+    const Pair *const twf_l = TWL[0], *const twi_l = TWL[1];
 
     // --- finish the scan: b_poly and the column tags, before any packing work
     if (tid == 0) { bad_column = 0x7fffffff; rejected = 0; }
@@ -346,7 +354,7 @@ void k_chain(CtxDev c, BatchDev bt) {
             for (int r = 0; r < 2; r++) {
                 const int e = tid + r * (N / 4);
                 uint64_t x = A[lds_ix(e)], y = A[lds_ix(e + N / 2)];
-                ct(x, y, twf[1]);
+                ct(x, y, twf_l[1]);
                 A[lds_ix(e)] = x;
                 A[lds_ix(e + N / 2)] = y;
             }
@@ -358,7 +366,10 @@ void k_chain(CtxDev c, BatchDev bt) {
             uint64_t v[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) v[k] = A[lds_ix(base + (k << b))];
-            ct4<LOGN>(v, b, hi, twf);
+            if (b >= 2)
+                ct4<LOGN>(v, b, hi, twf_l);
+            else
+                ct4<LOGN>(v, b, hi, twf);
 #pragma unroll
             for (int k = 0; k < 4; k++) A[lds_ix(base + (k << b))] = v[k];
             __syncthreads();
@@ -380,8 +391,8 @@ void k_chain(CtxDev c, BatchDev bt) {
 
     // --- backward recursion with the online gather
     for (int i = gamma - 2; i >= 0; i--) {
-        inverse_to<LOGN>(X, C, (LOGN & 1) ? LOGN - 1 : LOGN - 2, twi);
-        middle<LOGN, EXACT>(C, D, twi, twf);
+        inverse_to<LOGN>(X, C, (LOGN & 1) ? LOGN - 1 : LOGN - 2, twi, twi_l);
+        middle<LOGN, EXACT>(C, D, twi_l, twf_l);
         // last forward pass (bits 1-0) on own slots, then the pointwise work
         uint64_t t0[4], t1[4];
         const int base = 4 * tid;
@@ -430,8 +441,8 @@ void k_chain(CtxDev c, BatchDev bt) {
         }
         __syncthreads();
         for (int b = 2; b < LOGN - 1; b += 2) {
-            lds_inv_pass<LOGN>(C, b, twi);
-            lds_inv_pass<LOGN>(D, b, twi);
+            lds_inv_pass<LOGN>(C, b, twi_l);
+            lds_inv_pass<LOGN>(D, b, twi_l);
             __syncthreads();
         }
         if constexpr (LOGN & 1) {
@@ -439,11 +450,11 @@ void k_chain(CtxDev c, BatchDev bt) {
             for (int r = 0; r < 2; r++) {
                 const int e = tid + r * (N / 4);
                 uint64_t x = C[lds_ix(e)], y = C[lds_ix(e + N / 2)];
-                gs(x, y, twi[1]);
+                gs(x, y, twi_l[1]);
                 C[lds_ix(e)] = canon4(x);
                 C[lds_ix(e + N / 2)] = canon4(y);
                 x = D[lds_ix(e)], y = D[lds_ix(e + N / 2)];
-                gs(x, y, twi[1]);
+                gs(x, y, twi_l[1]);
                 D[lds_ix(e)] = canon4(x);
                 D[lds_ix(e + N / 2)] = canon4(y);
             }
