@@ -137,11 +137,22 @@ __device__ __forceinline__ void lds_fwd_pass2(uint64_t *x, uint64_t *y, int b, c
 // Inverse transform of the register state s (own slots) into LDS x, up to and not
 // including the passes that the caller fuses. Leaves x holding the values after the
 // inverse passes over bits [0, stop), with a barrier at the end.
+// warm receives one word loaded from warm_src, issued right behind the own-slot pass's
+// twiddle loads: a cache warm-up whose wait the caller places late.
 template <int LOGN>
-__device__ __forceinline__ void inverse_to(uint64_t s[4], uint64_t *x, int stop, const Pair *tw, const Pair *tw_l) {
+__device__ __forceinline__ void inverse_to(uint64_t s[4], uint64_t *x, int stop, const Pair *tw, const Pair *tw_l,
+                                           const uint32_t *warm_src, uint32_t &warm) {
+    constexpr int N = 1 << LOGN;
+    const int t = threadIdx.x;
+    const Pair w0 = tw[N / 2 + 2 * t], w1 = tw[N / 2 + 2 * t + 1], w2 = tw[N / 4 + t];
+    warm = *warm_src;
     uint64_t v[4] = {s[0], s[1], s[2], s[3]};
-    gs4<LOGN>(v, 0, threadIdx.x, tw);
-    const int base = 4 * threadIdx.x;
+    // the radix-4 inverse pass over bits 0-1 (gs4 at b = 0)
+    gs(v[0], v[1], w0);
+    gs(v[2], v[3], w1);
+    gs(v[0], v[2], w2);
+    gs(v[1], v[3], w2);
+    const int base = 4 * t;
 #pragma unroll
     for (int k = 0; k < 4; k++) x[lds_ix(base + k)] = v[k];
     step_sync(stop > 2);  // wave-local when the pass over bits 2-3 follows
@@ -391,7 +402,13 @@ void k_chain(CtxDev c, BatchDev bt) {
 
     // --- backward recursion with the online gather
     for (int i = gamma - 2; i >= 0; i--) {
-        inverse_to<LOGN>(X, C, (LOGN & 1) ? LOGN - 1 : LOGN - 2, twi, twi_l);
+        const Pair *S = sh.store + (size_t)i * N;
+        // this step's store row is read at the end of the step; touching its lines now (one
+        // word per thread, unconditionally, so the wait counts stay exact) lets the HBM fetch
+        // overlap the transforms
+        uint32_t warm = 0;
+        inverse_to<LOGN>(X, C, (LOGN & 1) ? LOGN - 1 : LOGN - 2, twi, twi_l,
+                         reinterpret_cast<const uint32_t *>(S + 4 * tid), warm);
         middle<LOGN, EXACT>(C, D, twi_l, twf_l);
         // last forward pass (bits 1-0) on own slots, then the pointwise work
         uint64_t t0[4], t1[4];
@@ -403,8 +420,8 @@ void k_chain(CtxDev c, BatchDev bt) {
         }
         ct4<LOGN>(t0, 0, tid, twf);
         ct4<LOGN>(t1, 0, tid, twf);
+        asm volatile("" ::"v"(warm));  // the warm-up load completes here, long after its issue
         const uint32_t t = c.gpow[i];
-        const Pair *S = sh.store + (size_t)i * N;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int j = base + k;
