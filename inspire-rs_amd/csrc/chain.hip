@@ -291,12 +291,16 @@ void k_chain(CtxDev c, BatchDev bt) {
     const Pair *Y = bt.yprimes ? bt.yprimes[qi] : bt.yprime;
     // Twiddles of every pass above the own-slot one (ladder entries below n/4: 16 KiB at
     // n = 2048) live in LDS for the whole kernel; the own-slot passes read theirs from L2.
-    __shared__ Pair TWL[2][(1 << LOGN) / 4];
-    for (int m = tid; m < N / 4; m += T) {
-        TWL[0][m] = twf[m];
-        TWL[1][m] = twi[m];
+    // The three-per-CU shape needs that LDS for its third workgroup and reads them from L2.
+    constexpr int TWL_N = OCC >= 6 ? 1 : (1 << LOGN) / 4;
+    __shared__ Pair TWL[2][TWL_N];
+    if constexpr (OCC < 6) {
+        for (int m = tid; m < N / 4; m += T) {
+            TWL[0][m] = twf[m];
+            TWL[1][m] = twi[m];
+        }
     }
-    const Pair *const twf_l = TWL[0], *const twi_l = TWL[1];
+    const Pair *const twf_l = OCC >= 6 ? twf : TWL[0], *const twi_l = OCC >= 6 ? twi : TWL[1];
 
     // --- finish the scan: b_poly and the column tags, before any packing work
     if (tid == 0) { bad_column = 0x7fffffff; rejected = 0; }
