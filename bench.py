@@ -49,9 +49,38 @@ def make_batches(rng, count, batch, num_shards, q):
     return ids, seeds, b
 
 
+def _cpu_threads():
+    """Host threads for the all-core CPU legs: the box's share (at most 16 per GPU)."""
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
+def _timed_queries(run_one, budget_s, threads):
+    """Queries per second of run_one(worker_rng) on `threads` host threads for about budget_s
+    (the oracle's C calls release the GIL, so the threads run in parallel)."""
+    import threading
+
+    counts = [0] * threads
+    t0 = time.perf_counter()
+
+    def worker(k):
+        rng = np.random.default_rng(1000 + k)
+        while time.perf_counter() - t0 < budget_s or counts[k] == 0:
+            run_one(rng)
+            counts[k] += 1
+
+    ts = [threading.Thread(target=worker, args=(k,)) for k in range(threads)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    dt = time.perf_counter() - t0
+    return sum(counts), dt
+
+
 def cpu_baseline(budget_s=10.0):
-    """The CPU oracle (C restatement of the reference respond, one thread) on a
-    bounded sample: seeded d2048 gamma=256 queries against a prebuilt store."""
+    """The CPU oracle (C restatement of the reference respond) on a bounded sample: seeded
+    d2048 gamma=256 queries against a prebuilt store, on all of the box's host cores (one
+    query per thread, as a multi-threaded server answers independent requests) and on one."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
@@ -61,16 +90,19 @@ def cpu_baseline(budget_s=10.0):
     store = p.store_build(cols)
     y = rng.integers(0, oracle.Q, (3, N), dtype=np.uint64)
     w_seed = bytes(32)
-    done, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s or done == 0:
-        seed = rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
-        b = rng.integers(0, oracle.Q, N, dtype=np.uint64)
-        a = oracle.expand_seed(seed, N)
-        p.respond(w_seed, a, b, cols, y, store)
-        done += 1
-    dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "queries/s", "cores": 1, "kind": "port",
-            "sample": f"{done} seeded d2048 gamma=256 queries against a prebuilt store (C restatement, 1 thread), {dt:.1f} s"}
+
+    def one(r):
+        seed = r.integers(0, 256, 32, dtype=np.uint8).tobytes()
+        b = r.integers(0, oracle.Q, N, dtype=np.uint64)
+        p.respond(w_seed, oracle.expand_seed(seed, N), b, cols, y, store)
+
+    n1, dt1 = _timed_queries(one, budget_s / 2, 1)
+    T = _cpu_threads()
+    nT, dtT = _timed_queries(one, budget_s / 2, T)
+    return {"value": round(nT / dtT, 3), "unit": "queries/s", "cores": T, "kind": "port",
+            "single_core": round(n1 / dt1, 3),
+            "sample": f"{nT} seeded d2048 gamma=256 queries against a prebuilt store (C restatement), "
+                      f"{T} threads for {dtT:.1f} s; and {n1} on 1 thread for {dt1:.1f} s"}
 
 
 def served(g, seconds, rate_qps, seed=7):
@@ -167,8 +199,9 @@ def sm_cells(rng, rows):
 
 
 def sm_cpu_baseline(budget_s):
-    """The CPU oracle's special-modulus respond (C restatement of respond_impl, one thread) on a
-    bounded sample of served-set queries against one prebuilt table and seat."""
+    """The CPU oracle's special-modulus respond (C restatement of respond_impl) on a bounded
+    sample of served-set queries against one prebuilt table and seat, on all of the box's
+    host cores (one query per thread) and on one."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
@@ -177,14 +210,17 @@ def sm_cpu_baseline(budget_s):
     table = ring.table(sm_cells(rng, SM_N))
     seat = ring.seat(rng.integers(0, oracle.SM_Q, SM_N, dtype=np.uint32),
                      rng.integers(0, oracle.SM_P, SM_N, dtype=np.uint32), bytes(32))
-    done, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s or done == 0:
-        ring.respond(table, seat, rng.bytes(32), rng.integers(0, oracle.SM_Q, SM_N, dtype=np.uint32))
-        done += 1
-    dt = time.perf_counter() - t0
-    return {"value": round(done / dt, 3), "unit": "queries/s", "cores": 1, "kind": "port",
-            "sample": f"{done} ring2048_cell256 queries against one prebuilt table (C restatement, 1 thread), "
-                      f"{dt:.1f} s"}
+
+    def one(r):
+        ring.respond(table, seat, r.bytes(32), r.integers(0, oracle.SM_Q, SM_N, dtype=np.uint32))
+
+    n1, dt1 = _timed_queries(one, budget_s / 2, 1)
+    T = _cpu_threads()
+    nT, dtT = _timed_queries(one, budget_s / 2, T)
+    return {"value": round(nT / dtT, 3), "unit": "queries/s", "cores": T, "kind": "port",
+            "single_core": round(n1 / dt1, 3),
+            "sample": f"{nT} ring2048_cell256 queries against one prebuilt table (C restatement), "
+                      f"{T} threads for {dtT:.1f} s; and {n1} on 1 thread for {dt1:.1f} s"}
 
 
 def special_modulus(g, torch, dev, stream, args):
@@ -568,7 +604,6 @@ def main():
     if not args.no_cpu:
         log("bench: CPU baseline")
         cpu = cpu_baseline(args.cpu_seconds)
-        cpu["cores"] = 1
         if sm is not None:
             sm["cpu_baseline"] = sm_cpu_baseline(min(args.cpu_seconds, 10.0))
 
