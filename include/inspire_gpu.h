@@ -58,7 +58,9 @@ void inspire_ctx_destroy(inspire_ctx *ctx);
 uint32_t inspire_ctx_key_rows(const inspire_ctx *ctx);
 /* Launch shaping of the packing kernel (no reference counterpart; results are
  * unchanged): 1 (default) gives every query a CU of its own, 2 lets two queries share
- * a CU, which raises throughput when far more queries are in flight than CUs. */
+ * a CU, which raises throughput when far more queries are in flight than CUs, and 3
+ * runs a kernel built for three per CU (fewer registers). inspire_ctx_autotune picks
+ * one by timing. */
 int inspire_ctx_set_queries_per_cu(inspire_ctx *ctx, uint32_t queries_per_cu, char *err, size_t err_cap);
 
 /* Upload one shard of the EncodedDatabase (ShardData, src/pir/setup.rs:280-289).
