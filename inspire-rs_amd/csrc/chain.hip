@@ -104,9 +104,9 @@ __device__ __forceinline__ void group_of(int g, int b, int &base, int &hi) {
 }
 
 template <int LOGN>
-__device__ __forceinline__ void lds_inv_pass(uint64_t *x, int b, const Pair *tw) {
+__device__ __forceinline__ void lds_inv_pass(uint64_t *x, int b, const Pair *tw, int t) {
     int base, hi;
-    group_of(threadIdx.x, b, base, hi);
+    group_of(t, b, base, hi);
     uint64_t v[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) v[k] = x[lds_ix(base + (k << b))];
@@ -116,9 +116,9 @@ __device__ __forceinline__ void lds_inv_pass(uint64_t *x, int b, const Pair *tw)
 }
 
 template <int LOGN>
-__device__ __forceinline__ void lds_fwd_pass2(uint64_t *x, uint64_t *y, int b, const Pair *tw) {
+__device__ __forceinline__ void lds_fwd_pass2(uint64_t *x, uint64_t *y, int b, const Pair *tw, int t) {
     int base, hi;
-    group_of(threadIdx.x, b, base, hi);
+    group_of(t, b, base, hi);
     uint64_t u[4], v[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -141,9 +141,8 @@ __device__ __forceinline__ void lds_fwd_pass2(uint64_t *x, uint64_t *y, int b, c
 // twiddle loads: a cache warm-up whose wait the caller places late.
 template <int LOGN>
 __device__ __forceinline__ void inverse_to(uint64_t s[4], uint64_t *x, int stop, const Pair *tw, const Pair *tw_l,
-                                           const uint32_t *warm_src, uint32_t &warm) {
+                                           const uint32_t *warm_src, uint32_t &warm, int t) {
     constexpr int N = 1 << LOGN;
-    const int t = threadIdx.x;
     const Pair w0 = tw[N / 2 + 2 * t], w1 = tw[N / 2 + 2 * t + 1], w2 = tw[N / 4 + t];
     warm = *warm_src;
     uint64_t v[4] = {s[0], s[1], s[2], s[3]};
@@ -157,7 +156,7 @@ __device__ __forceinline__ void inverse_to(uint64_t s[4], uint64_t *x, int stop,
     for (int k = 0; k < 4; k++) x[lds_ix(base + k)] = v[k];
     step_sync(stop > 2);  // wave-local when the pass over bits 2-3 follows
     for (int b = 2; b < stop; b += 2) {
-        lds_inv_pass<LOGN>(x, b, tw_l);
+        lds_inv_pass<LOGN>(x, b, tw_l, t);
         // the next pass is over bits b+2, b+3, or the top of the transform
         step_sync(b + 4 <= 8 && b + 2 < stop);
     }
@@ -182,13 +181,13 @@ __device__ __forceinline__ void digits(uint64_t c, uint64_t &d0, uint64_t &d1) {
 // bits 3-2. On return C and D hold the digit transforms after all passes but the
 // last (bits 1-0), barrier included.
 template <int LOGN, bool EXACT>
-__device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi, const Pair *twf) {
+__device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi, const Pair *twf, int t) {
     constexpr int N = 1 << LOGN;
     if constexpr (LOGN & 1) {
         // single top bit: pairs (e, e + N/2), two per thread
 #pragma unroll
         for (int r = 0; r < 2; r++) {
-            const int e = threadIdx.x + r * (N / 4);
+            const int e = t + r * (N / 4);
             uint64_t x = C[lds_ix(e)], y = C[lds_ix(e + N / 2)];
             gs(x, y, twi[1]);
             uint64_t a0, a1, b0, b1;
@@ -203,14 +202,14 @@ __device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi
         }
         __syncthreads();
         for (int b = LOGN - 3; b >= 2; b -= 2) {
-            lds_fwd_pass2<LOGN>(C, D, b, twf);
+            lds_fwd_pass2<LOGN>(C, D, b, twf, t);
             step_sync(b + 2 <= 8);  // the next pass is over bits b-2, b-1 (or the own slots)
         }
     } else {
         // top two bits: one radix-4 group per thread
         int base, hi;
         const int b = LOGN - 2;
-        group_of(threadIdx.x, b, base, hi);
+        group_of(t, b, base, hi);
         uint64_t v[4], u[4], w[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) v[k] = C[lds_ix(base + (k << b))];
@@ -227,7 +226,7 @@ __device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi
         __syncthreads();
 #pragma unroll
         for (int bb = LOGN - 4; bb >= 2; bb -= 2) {
-            lds_fwd_pass2<LOGN>(C, D, bb, twf);
+            lds_fwd_pass2<LOGN>(C, D, bb, twf, t);
             step_sync(bb + 2 <= 8);
         }
     }
@@ -406,24 +405,30 @@ void k_chain(CtxDev c, BatchDev bt) {
 
     // --- backward recursion with the online gather
     for (int i = gamma - 2; i >= 0; i--) {
+        // the thread index is re-read through an empty asm each step in the 6-wave build: its
+        // derived addresses are then recomputed in the step instead of being hoisted out of the
+        // loop into registers, which the 80-register budget cannot hold (74 registers, no
+        // spills, against 80 with 45 spills when hoisted)
+        int tl = tid;
+        if constexpr (OCC >= 6) asm volatile("" : "+v"(tl));
         const Pair *S = sh.store + (size_t)i * N;
         // this step's store row is read at the end of the step; touching its lines now (one
         // word per thread, unconditionally, so the wait counts stay exact) lets the HBM fetch
         // overlap the transforms
         uint32_t warm = 0;
         inverse_to<LOGN>(X, C, (LOGN & 1) ? LOGN - 1 : LOGN - 2, twi, twi_l,
-                         reinterpret_cast<const uint32_t *>(S + 4 * tid), warm);
-        middle<LOGN, EXACT>(C, D, twi_l, twf_l);
+                         reinterpret_cast<const uint32_t *>(S + 4 * tl), warm, tl);
+        middle<LOGN, EXACT>(C, D, twi_l, twf_l, tl);
         // last forward pass (bits 1-0) on own slots, then the pointwise work
         uint64_t t0[4], t1[4];
-        const int base = 4 * tid;
+        const int base = 4 * tl;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             t0[k] = C[lds_ix(base + k)];
             t1[k] = D[lds_ix(base + k)];
         }
-        ct4<LOGN>(t0, 0, tid, twf);
-        ct4<LOGN>(t1, 0, tid, twf);
+        ct4<LOGN>(t0, 0, tl, twf);
+        ct4<LOGN>(t1, 0, tl, twf);
         asm volatile("" ::"v"(warm));  // the warm-up load completes here, long after its issue
         const uint32_t t = c.gpow[i];
 #pragma unroll
@@ -462,8 +467,8 @@ void k_chain(CtxDev c, BatchDev bt) {
         }
         __syncthreads();
         for (int b = 2; b < LOGN - 1; b += 2) {
-            lds_inv_pass<LOGN>(C, b, twi_l);
-            lds_inv_pass<LOGN>(D, b, twi_l);
+            lds_inv_pass<LOGN>(C, b, twi_l, tid);
+            lds_inv_pass<LOGN>(D, b, twi_l, tid);
             __syncthreads();
         }
         if constexpr (LOGN & 1) {
