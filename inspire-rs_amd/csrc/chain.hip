@@ -134,17 +134,17 @@ __device__ __forceinline__ void lds_fwd_pass2(uint64_t *x, uint64_t *y, int b, c
     }
 }
 
-// Inverse transform of the register state s (own slots) into LDS x, up to and not
-// including the passes that the caller fuses. Leaves x holding the values after the
-// inverse passes over bits [0, stop), with a barrier at the end.
-// warm receives one word loaded from warm_src, issued right behind the own-slot pass's
-// twiddle loads: a cache warm-up whose wait the caller places late.
-template <int LOGN>
-__device__ __forceinline__ void inverse_to(uint64_t s[4], uint64_t *x, int stop, const Pair *tw, const Pair *tw_l,
+// Inverse transform of the register state s (own slots) into LDS x over bits [0, STOP), with
+// a barrier at the end. Passes at base bit BL and up read their twiddles from the LDS copy
+// tw_l, lower ones from tw (L2). warm receives one word loaded from warm_src, issued right
+// behind the last L2 twiddle loads of the transform: a cache warm-up whose wait the caller
+// places late (vector loads complete in order, so no vector load may come between).
+template <int LOGN, int STOP, int BL>
+__device__ __forceinline__ void inverse_to(uint64_t s[4], uint64_t *x, const Pair *tw, const Pair *tw_l,
                                            const uint32_t *warm_src, uint32_t &warm, int t) {
     constexpr int N = 1 << LOGN;
     const Pair w0 = tw[N / 2 + 2 * t], w1 = tw[N / 2 + 2 * t + 1], w2 = tw[N / 4 + t];
-    warm = *warm_src;
+    if constexpr (BL <= 2) warm = *warm_src;
     uint64_t v[4] = {s[0], s[1], s[2], s[3]};
     // the radix-4 inverse pass over bits 0-1 (gs4 at b = 0)
     gs(v[0], v[1], w0);
@@ -154,11 +154,31 @@ __device__ __forceinline__ void inverse_to(uint64_t s[4], uint64_t *x, int stop,
     const int base = 4 * t;
 #pragma unroll
     for (int k = 0; k < 4; k++) x[lds_ix(base + k)] = v[k];
-    step_sync(stop > 2);  // wave-local when the pass over bits 2-3 follows
-    for (int b = 2; b < stop; b += 2) {
+    step_sync(STOP > 2);  // wave-local when the pass over bits 2-3 follows
+    int b = 2;
+    if constexpr (BL > 2) {
+        // the pass over bits 2-3 reads L2 twiddles: its twiddle loads, then the warm-up
+        int gb, hi;
+        group_of(t, 2, gb, hi);
+        const int m0 = N >> 3, m1 = N >> 4;
+        const Pair u0 = tw[m0 + 2 * hi], u1 = tw[m0 + 2 * hi + 1], u2 = tw[m1 + hi];
+        warm = *warm_src;
+        uint64_t y[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) y[k] = x[lds_ix(gb + (k << 2))];
+        gs(y[0], y[1], u0);
+        gs(y[2], y[3], u1);
+        gs(y[0], y[2], u2);
+        gs(y[1], y[3], u2);
+#pragma unroll
+        for (int k = 0; k < 4; k++) x[lds_ix(gb + (k << 2))] = y[k];
+        step_sync(6 <= 8 && 4 < STOP);
+        b = 4;
+    }
+    for (; b < STOP; b += 2) {
         lds_inv_pass<LOGN>(x, b, tw_l, t);
         // the next pass is over bits b+2, b+3, or the top of the transform
-        step_sync(b + 4 <= 8 && b + 2 < stop);
+        step_sync(b + 4 <= 8 && b + 2 < STOP);
     }
 }
 
@@ -180,8 +200,9 @@ __device__ __forceinline__ void digits(uint64_t c, uint64_t &d0, uint64_t &d1) {
 // transforms for this thread's share, then the remaining forward passes down to
 // bits 3-2. On return C and D hold the digit transforms after all passes but the
 // last (bits 1-0), barrier included.
-template <int LOGN, bool EXACT>
-__device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi, const Pair *twf, int t) {
+template <int LOGN, bool EXACT, int BL>
+__device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi, const Pair *twf, const Pair *twf_g,
+                                       int t) {
     constexpr int N = 1 << LOGN;
     if constexpr (LOGN & 1) {
         // single top bit: pairs (e, e + N/2), two per thread
@@ -202,7 +223,10 @@ __device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi
         }
         __syncthreads();
         for (int b = LOGN - 3; b >= 2; b -= 2) {
-            lds_fwd_pass2<LOGN>(C, D, b, twf, t);
+            if (b >= BL)
+                lds_fwd_pass2<LOGN>(C, D, b, twf, t);
+            else
+                lds_fwd_pass2<LOGN>(C, D, b, twf_g, t);
             step_sync(b + 2 <= 8);  // the next pass is over bits b-2, b-1 (or the own slots)
         }
     } else {
@@ -224,9 +248,11 @@ __device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi
             D[lds_ix(base + (k << b))] = w[k];
         }
         __syncthreads();
-#pragma unroll
         for (int bb = LOGN - 4; bb >= 2; bb -= 2) {
-            lds_fwd_pass2<LOGN>(C, D, bb, twf, t);
+            if (bb >= BL)
+                lds_fwd_pass2<LOGN>(C, D, bb, twf, t);
+            else
+                lds_fwd_pass2<LOGN>(C, D, bb, twf_g, t);
             step_sync(bb + 2 <= 8);
         }
     }
@@ -271,35 +297,41 @@ __device__ inline uint32_t wire_byte(const WireLayout &L, const uint64_t *C, con
     return 0;  // PackingMode::Inspiring (u32 0) and the padding of the last word
 }
 
-// OCC is the number of waves per SIMD the register allocation targets: 4 for one or two
-// queries per CU, 6 for three (three workgroups' LDS fit a CU; registers then cap at 80).
-template <int LOGN, int ROWS, bool EXACT, int OCC>
-__global__ __launch_bounds__((1 << LOGN) / 4) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
+// Registers are budgeted for 6 waves per SIMD (80 per lane), which lets three workgroups share
+// a CU. The same build is also the fastest with one or two queries per CU (measured: 13%
+// less time per query alone on a CU than a 128-register build): nothing is hoisted out of
+// the recursion that the step can recompute.
+template <int LOGN, int ROWS, bool EXACT>
+__global__ __launch_bounds__((1 << LOGN) / 4) __attribute__((amdgpu_waves_per_eu(6, 8)))
 void k_chain(CtxDev c, BatchDev bt) {
     constexpr int N = 1 << LOGN;
     constexpr int T = N / 4;
-    __shared__ uint64_t A[LdsSize<LOGN>::value];  // NTT(a)
-    __shared__ uint64_t C[LdsSize<LOGN>::value];  // transform buffer / digit 0
-    __shared__ uint64_t D[LdsSize<LOGN>::value];  // digit 1
-    __shared__ int bad_column;
-    __shared__ int rejected;
+    // Three workgroups per CU must fit 160 KiB of LDS (53 KiB each at n = 2048): NTT(a)
+    // unpadded (it is only gathered from), LDS twiddles for the passes at bit 4 and up only,
+    // and the prologue's two flags in the first words of C, which nothing else touches before
+    // the recursion.
+    constexpr int BL = 4;                    // lowest pass base bit with LDS twiddles
+    constexpr int TWN = (1 << LOGN) >> BL;   // ladder entries below n >> BL
+    __shared__ uint64_t A[N];                      // NTT(a), unpadded
+    __shared__ uint64_t C[LdsSize<LOGN>::value];   // transform buffer / digit 0
+    __shared__ uint64_t D[LdsSize<LOGN>::value];   // digit 1
+    int &bad_column = reinterpret_cast<int *>(C)[0];
+    int &rejected = reinterpret_cast<int *>(C)[1];
+    auto aix = [](int e) { return e; };
 
     const int qi = blockIdx.x, tid = threadIdx.x, gamma = c.gamma;
     const ShardDev sh = c.shards[bt.shard_ids[qi]];
     const Pair *twf = c.twf, *twi = c.twi;
     const Pair *Y = bt.yprimes ? bt.yprimes[qi] : bt.yprime;
-    // Twiddles of every pass above the own-slot one (ladder entries below n/4: 16 KiB at
-    // n = 2048) live in LDS for the whole kernel; the own-slot passes read theirs from L2.
-    // The three-per-CU shape needs that LDS for its third workgroup and reads them from L2.
-    constexpr int TWL_N = OCC >= 6 ? 1 : (1 << LOGN) / 4;
-    __shared__ Pair TWL[2][TWL_N];
-    if constexpr (OCC < 6) {
-        for (int m = tid; m < N / 4; m += T) {
-            TWL[0][m] = twf[m];
-            TWL[1][m] = twi[m];
-        }
+    // Twiddles of the passes at base bit BL and up (ladder entries below n >> BL: 16 KiB at
+    // n = 2048 for BL = 2, 4 KiB for BL = 4) live in LDS for the whole kernel; the lower
+    // passes read theirs from L2.
+    __shared__ Pair TWL[2][TWN];
+    for (int m = tid; m < TWN; m += T) {
+        TWL[0][m] = twf[m];
+        TWL[1][m] = twi[m];
     }
-    const Pair *const twf_l = OCC >= 6 ? twf : TWL[0], *const twi_l = OCC >= 6 ? twi : TWL[1];
+    const Pair *const twf_l = TWL[0], *const twi_l = TWL[1];
 
     // --- finish the scan: b_poly and the column tags, before any packing work
     if (tid == 0) { bad_column = 0x7fffffff; rejected = 0; }
@@ -326,7 +358,7 @@ void k_chain(CtxDev c, BatchDev bt) {
 
     // --- a: the query's expanded row, or gen_range(0..q) draws from ChaCha20(seed)
     if (bt.a_in) {
-        for (int j = tid; j < N; j += T) A[lds_ix(j)] = bt.a_in[(size_t)qi * N + j];
+        for (int j = tid; j < N; j += T) A[aix(j)] = bt.a_in[(size_t)qi * N + j];
     } else {
         uint32_t key[8];
         seed_key(bt.seeds + (size_t)qi * 32, key);
@@ -337,7 +369,7 @@ void k_chain(CtxDev c, BatchDev bt) {
             for (int u = 0; u < 8; u++) {
                 uint64_t v = (uint64_t)w[2 * u] | (uint64_t)w[2 * u + 1] << 32, a;
                 if (!uniform_q_draw(v, a)) rejected = 1;
-                A[lds_ix(blk * 8 + u)] = a;
+                A[aix(blk * 8 + u)] = a;
             }
         }
         __syncthreads();
@@ -355,7 +387,7 @@ void k_chain(CtxDev c, BatchDev bt) {
                     if (word >= 16) { chacha20_block(key, blk++, 0, w); word = 0; }
                     hi32 = w[word++];
                     uint64_t a;
-                    if (uniform_q_draw((uint64_t)lo32 | (uint64_t)hi32 << 32, a)) A[lds_ix(got++)] = a;
+                    if (uniform_q_draw((uint64_t)lo32 | (uint64_t)hi32 << 32, a)) A[aix(got++)] = a;
                 }
             }
         }
@@ -367,10 +399,10 @@ void k_chain(CtxDev c, BatchDev bt) {
 #pragma unroll
             for (int r = 0; r < 2; r++) {
                 const int e = tid + r * (N / 4);
-                uint64_t x = A[lds_ix(e)], y = A[lds_ix(e + N / 2)];
+                uint64_t x = A[aix(e)], y = A[aix(e + N / 2)];
                 ct(x, y, twf_l[1]);
-                A[lds_ix(e)] = x;
-                A[lds_ix(e + N / 2)] = y;
+                A[aix(e)] = x;
+                A[aix(e + N / 2)] = y;
             }
             __syncthreads();
         }
@@ -379,13 +411,13 @@ void k_chain(CtxDev c, BatchDev bt) {
             group_of(tid, b, base, hi);
             uint64_t v[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++) v[k] = A[lds_ix(base + (k << b))];
-            if (b >= 2)
+            for (int k = 0; k < 4; k++) v[k] = A[aix(base + (k << b))];
+            if (b >= BL)
                 ct4<LOGN>(v, b, hi, twf_l);
             else
                 ct4<LOGN>(v, b, hi, twf);
 #pragma unroll
-            for (int k = 0; k < 4; k++) A[lds_ix(base + (k << b))] = v[k];
+            for (int k = 0; k < 4; k++) A[aix(base + (k << b))] = v[k];
             __syncthreads();
         }
     }
@@ -398,27 +430,27 @@ void k_chain(CtxDev c, BatchDev bt) {
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int j = 4 * tid + k;
-            X[k] = shoup(A[lds_ix(automorph_slot(t, j, LOGN))], S[j]);
+            X[k] = shoup(A[aix(automorph_slot(t, j, LOGN))], S[j]);
             acc[k] = 0;
         }
     }
 
     // --- backward recursion with the online gather
     for (int i = gamma - 2; i >= 0; i--) {
-        // the thread index is re-read through an empty asm each step in the 6-wave build: its
-        // derived addresses are then recomputed in the step instead of being hoisted out of the
-        // loop into registers, which the 80-register budget cannot hold (74 registers, no
-        // spills, against 80 with 45 spills when hoisted)
+        // the thread index is re-read through an empty asm each step: its derived addresses
+        // are then recomputed in the step instead of being hoisted out of the loop into
+        // registers, which the 80-register budget cannot hold (no spills, against 45 spilled
+        // registers when hoisted)
         int tl = tid;
-        if constexpr (OCC >= 6) asm volatile("" : "+v"(tl));
+        asm volatile("" : "+v"(tl));
         const Pair *S = sh.store + (size_t)i * N;
         // this step's store row is read at the end of the step; touching its lines now (one
         // word per thread, unconditionally, so the wait counts stay exact) lets the HBM fetch
         // overlap the transforms
         uint32_t warm = 0;
-        inverse_to<LOGN>(X, C, (LOGN & 1) ? LOGN - 1 : LOGN - 2, twi, twi_l,
+        inverse_to<LOGN, (LOGN & 1) ? LOGN - 1 : LOGN - 2, BL>(X, C, twi, twi_l,
                          reinterpret_cast<const uint32_t *>(S + 4 * tl), warm, tl);
-        middle<LOGN, EXACT>(C, D, twi_l, twf_l, tl);
+        middle<LOGN, EXACT, BL>(C, D, twi_l, twf_l, twf, tl);
         // last forward pass (bits 1-0) on own slots, then the pointwise work
         uint64_t t0[4], t1[4];
         const int base = 4 * tl;
@@ -439,7 +471,7 @@ void k_chain(CtxDev c, BatchDev bt) {
             // rows (W_2 = kx*w_2, W_0 = w_0 + k0*w_2, W_1 = w_1 + k1*w_2; same for y), so its
             // operand here is R itself
             const uint64_t d[3] = {t0[k], t1[k], X[k]};
-            uint64_t r = shoup(A[lds_ix(src)], S[j]);
+            uint64_t r = shoup(A[aix(src)], S[j]);
             uint64_t o = acc[k];
 #pragma unroll
             for (int rr = 0; rr < ROWS; rr++) {
@@ -467,8 +499,13 @@ void k_chain(CtxDev c, BatchDev bt) {
         }
         __syncthreads();
         for (int b = 2; b < LOGN - 1; b += 2) {
-            lds_inv_pass<LOGN>(C, b, twi_l, tid);
-            lds_inv_pass<LOGN>(D, b, twi_l, tid);
+            if (b >= BL) {
+                lds_inv_pass<LOGN>(C, b, twi_l, tid);
+                lds_inv_pass<LOGN>(D, b, twi_l, tid);
+            } else {
+                lds_inv_pass<LOGN>(C, b, twi, tid);
+                lds_inv_pass<LOGN>(D, b, twi, tid);
+            }
             __syncthreads();
         }
         if constexpr (LOGN & 1) {
@@ -536,15 +573,13 @@ void k_chain(CtxDev c, BatchDev bt) {
 // At most queries_per_cu workgroups per CU: a workgroup reserves more than
 // 1/(queries_per_cu + 1) of the CU's 160 KiB LDS (dynamic padding, never touched), so the
 // dispatcher spreads a launch over all CUs instead of stacking queries on some and
-// idling others. Three per CU use the kernel compiled for 6 waves per SIMD.
+// idling others.
 constexpr size_t kLdsPerCu = 160 * 1024;
 
 template <int LOGN>
 static hipError_t launch_chain_t(const CtxDev &c, const BatchDev &b, hipStream_t s) {
     constexpr int T = (1 << LOGN) / 4;
-    const bool occ6 = c.queries_per_cu >= 3;
-    auto kern = c.rows == 3 ? (occ6 ? k_chain<LOGN, 3, true, 6> : k_chain<LOGN, 3, true, 4>)
-                            : (occ6 ? k_chain<LOGN, 2, false, 6> : k_chain<LOGN, 2, false, 4>);
+    auto kern = c.rows == 3 ? k_chain<LOGN, 3, true> : k_chain<LOGN, 2, false>;
     hipFuncAttributes attr;
     hipError_t e = hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(kern));
     if (e != hipSuccess) return e;
