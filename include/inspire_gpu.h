@@ -102,7 +102,11 @@ void inspire_seat_destroy(inspire_seat *seat);
  * a_out + k*ring_dim receives a_hat, b_prefix_out + k*gamma the first gamma
  * coefficients of b (the packed response's prefix, src/pir/respond.rs:132-160).
  * switch_modulus = 0 returns residues mod q; otherwise each coefficient is switched
- * as mod_switch_poly (src/pir/mod_switch.rs:148-175) does. A refused query fails the
+ * as mod_switch_poly (src/pir/mod_switch.rs:148-175) does, for any target below 2^46
+ * (the 36- and 45-bit rungs, MOD_SWITCH_TARGET_36BIT/45BIT, mod_switch.rs:33,47). The
+ * switch is mod_switch_response_unchecked (:188); the noise gate of the checked form
+ * (check_mod_switch_noise_budget, :111-144) is a host f64 check made once per parameter
+ * set by the caller. A refused query fails the
  * whole batch with no partial output (src/pir/respond.rs:1185-1188). Host buffers. */
 int inspire_respond_batch(inspire_ctx *ctx, const inspire_seat *seat, const uint32_t *shard_ids,
                           const uint8_t *seeds, const uint64_t *b, size_t count, uint64_t switch_modulus,

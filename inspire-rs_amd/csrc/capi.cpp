@@ -407,8 +407,8 @@ static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_wo
         set_err(err, err_cap, "batch of %zu queries exceeds the workspace's %zu", count, ws->max_count);
         return INSPIRE_ERR_INVALID;
     }
-    if (switch_modulus >= (1ULL << 37)) {
-        set_err(err, err_cap, "switch modulus %llu is above the 37-bit targets this path implements",
+    if (switch_modulus >= kSwitchModulusLimit) {
+        set_err(err, err_cap, "switch modulus %llu is not below 2^46 (the 36- and 45-bit targets are implemented)",
                 (unsigned long long)switch_modulus);
         return INSPIRE_ERR_INVALID;
     }
@@ -536,8 +536,8 @@ int inspire_respond_graph_create(inspire_ctx *ctx, const inspire_seat *seat, ins
         set_err(err, err_cap, "batch of %zu queries does not fit the workspace's %zu", count, ws->max_count);
         return INSPIRE_ERR_INVALID;
     }
-    if (switch_modulus >= (1ULL << 37)) {
-        set_err(err, err_cap, "switch modulus %llu is above the 37-bit targets this path implements",
+    if (switch_modulus >= kSwitchModulusLimit) {
+        set_err(err, err_cap, "switch modulus %llu is not below 2^46 (the 36- and 45-bit targets are implemented)",
                 (unsigned long long)switch_modulus);
         return INSPIRE_ERR_INVALID;
     }
@@ -682,8 +682,8 @@ static int respond_host(inspire_ctx *ctx, const inspire_seat *seat, const uint32
                         (unsigned long long)kQ);
                 return INSPIRE_ERR_INVALID;
             }
-    if (switch_modulus >= (1ULL << 37)) {
-        set_err(err, err_cap, "switch modulus %llu is above the 37-bit targets this path implements",
+    if (switch_modulus >= kSwitchModulusLimit) {
+        set_err(err, err_cap, "switch modulus %llu is not below 2^46 (the 36- and 45-bit targets are implemented)",
                 (unsigned long long)switch_modulus);
         return INSPIRE_ERR_INVALID;
     }

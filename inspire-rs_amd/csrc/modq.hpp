@@ -104,7 +104,13 @@ __host__ __device__ __forceinline__ uint64_t mod_p61(uint64_t hi, uint64_t lo) {
     return v >= kTagPrime ? v - kTagPrime : v;
 }
 
-// floor((c * q' + floor(q/2)) / q) mod q' for c < q, q' < 2^37.
+// Switch moduli this path implements: every q' below 2^46, which covers the reference's
+// 36- and 45-bit targets (MOD_SWITCH_TARGET_36BIT/45BIT, src/pir/mod_switch.rs:33,47).
+constexpr uint64_t kSwitchModulusLimit = 1ULL << 46;
+
+// floor((c * q' + floor(q/2)) / q) mod q' for c < q, q' < 2^46. With X = c q' + floor(q/2) < 2^106,
+// e = floor(X / 2^60) is the quotient or one below it: X/q - X/2^60 = X (2^14 - 1) / (q 2^60)
+// < (q' + 1/2)(2^14 - 1) / 2^60 < 1. The quotient is at most q', so one subtraction reduces it.
 __host__ __device__ __forceinline__ uint64_t mod_switch_coeff(uint64_t c, uint64_t qn) {
     uint64_t hi, lo = mul_wide(c, qn, hi);
     const uint64_t half = kQ / 2;

@@ -350,8 +350,8 @@ int inspire_server_submit(inspire_server *sv, uint64_t handle, const uint32_t *s
         set_err(err, err_cap, "a request carries 1..=%zu queries, got %zu", sv->max_launch, count);
         return INSPIRE_ERR_INVALID;
     }
-    if (switch_modulus >= (1ULL << 37)) {
-        set_err(err, err_cap, "switch modulus %llu is above the 37-bit targets this path implements",
+    if (switch_modulus >= kSwitchModulusLimit) {
+        set_err(err, err_cap, "switch modulus %llu is not below 2^46 (the 36- and 45-bit targets are implemented)",
                 (unsigned long long)switch_modulus);
         return INSPIRE_ERR_INVALID;
     }

@@ -16,17 +16,27 @@
  *   generate_ksk_body                       src/inspiring/inspiring2.rs:891-929
  *   GadgetVector::power                     src/rgsw/types.rs:117-126
  *   ClientSession::seeded_row / query       src/pir/session.rs:399-433
- *   SeededRgswCiphertext::encrypt_with_rng  src/rgsw/types.rs:345-386
+ *   SeededRgswCiphertext::encrypt_with_rng  src/rgsw/types.rs:345-386 (orc_seeded_row)
  *   inverse_monomial                        src/pir/encode_db.rs:43-70
  *   ShardConfig::try_index_to_shard         src/params.rs:1367-1380
  *   encode_database / extract_column_value  src/pir/encode_db.rs:76-159
  *   the KAT cell (database bytes, seeds,    tests/packing_key_fixture/mod.rs:64-147
  *     probe query, next_query)
  *   ClientPackingKeys bincode               src/inspiring/inspiring2.rs:684-697, src/math/poly.rs:286-310
+ *   kept_mask_rows (approximate gadget:     src/inspiring/inspiring2.rs:839-856
+ *     stream gamma, lowest digit dropped)
+ *
+ * and the client's decode of a packed response (PIR.Extract):
+ *
+ *   RlweCiphertext::decrypt                 src/rlwe/enc.rs:91-107
+ *   PlaintextRounding::round                src/rlwe/enc.rs:14-44
+ *   mod_switch_secret_key                   src/pir/mod_switch.rs:221-252
+ *   reconstruct_entry                       src/pir/encode_db.rs:161-176
  *
  * Pinned by tests/golden/client_packing_key_kat.txt (key bytes and the sampler position
- * after them) and by the SESSIONKEY and RESP digests of
- * tests/golden/packing_seat_response_kat.txt.
+ * after them), by the SESSIONKEY and RESP digests of
+ * tests/golden/packing_seat_response_kat.txt, and (the decode) by the reference's
+ * post-switch error known answers at four target moduli (src/pir/mod_switch.rs:1070-1114).
  */
 #include "oracle.h"
 #include <stdlib.h>
@@ -35,7 +45,7 @@
 typedef unsigned __int128 u128;
 
 struct orc_cell {
-    size_t n, entry, gamma, records;
+    size_t n, entry, gamma, records, dropped;
     uint64_t q, delta;
     uint8_t *db;
     uint8_t w_seed[32];
@@ -79,11 +89,13 @@ static void negacyclic_mul(const orc_ntt *c, const uint64_t *a, const uint64_t *
     free(y);
 }
 
-/* ClientPackingKeys::generate_once for the exact gadget (base 2^20, 3 digits, none dropped):
- * y_body[r] = tau_g(s) * 2^(20 r) - s * w_mask[r] + e_r, the error rows drawn in row order. */
-int orc_client_packing_keys(size_t n, size_t gamma, const uint64_t *s, const uint8_t w_seed[32], orc_gauss *g,
-                            uint64_t *y_body) {
-    if (gamma == 0 || (gamma & (gamma - 1)) || gamma > n / 2) return -1;
+/* ClientPackingKeys::generate_once (base 2^20, 3 digits): the mask expands 3 rows from ChaCha20(w_seed)
+ * on stream 0 (exact gadget) or stream gamma (approximate gadget, which drops the lowest digit and
+ * keeps rows 1 and 2). Kept row r is y_body[r] = tau_g(s) * 2^(20 (dropped + r)) - s * w_mask[r] + e_r,
+ * the error rows drawn in row order. */
+int orc_client_packing_keys_gadget(size_t n, size_t gamma, size_t dropped, const uint64_t *s,
+                                   const uint8_t w_seed[32], orc_gauss *g, uint64_t *y_body) {
+    if (gamma == 0 || (gamma & (gamma - 1)) || gamma > n / 2 || dropped > 1) return -1;
     const uint64_t q = ORC_Q;
     orc_ntt ntt;
     if (orc_ntt_init(&ntt, n, q)) return -1;
@@ -92,16 +104,18 @@ int orc_client_packing_keys(size_t n, size_t gamma, const uint64_t *s, const uin
     orc_apply_automorphism(s, n, gen, q, tau);
     orc_chacha r;
     orc_chacha_from_seed(&r, w_seed);
-    orc_chacha_set_stream(&r, 0);
+    orc_chacha_set_stream(&r, dropped ? (uint64_t)gamma : 0);
     uint64_t power = 1;
-    for (size_t row = 0; row < 3; row++) {
+    for (size_t digit = 0; digit < 3; digit++) {
         for (size_t i = 0; i < n; i++) w[i] = orc_chacha_next_u64(&r) % q;
-        negacyclic_mul(&ntt, s, w, sw);
-        uint64_t *y = y_body + row * n;
-        for (size_t i = 0; i < n; i++) {
-            uint64_t e = from_signed(orc_gauss_sample(g), q);
-            uint64_t t = (uint64_t)(((u128)tau[i] * power) % q);
-            y[i] = (uint64_t)(((u128)t + (q - sw[i]) + e) % q);
+        if (digit >= dropped) {
+            negacyclic_mul(&ntt, s, w, sw);
+            uint64_t *y = y_body + (digit - dropped) * n;
+            for (size_t i = 0; i < n; i++) {
+                uint64_t e = from_signed(orc_gauss_sample(g), q);
+                uint64_t t = (uint64_t)(((u128)tau[i] * power) % q);
+                y[i] = (uint64_t)(((u128)t + (q - sw[i]) + e) % q);
+            }
         }
         power = (uint64_t)(((u128)power << 20) % q);
     }
@@ -110,6 +124,11 @@ int orc_client_packing_keys(size_t n, size_t gamma, const uint64_t *s, const uin
     free(sw);
     orc_ntt_free(&ntt);
     return 0;
+}
+
+int orc_client_packing_keys(size_t n, size_t gamma, const uint64_t *s, const uint8_t w_seed[32], orc_gauss *g,
+                            uint64_t *y_body) {
+    return orc_client_packing_keys_gadget(n, gamma, 0, s, w_seed, g, y_body);
 }
 
 static size_t put_u64(uint8_t *o, size_t at, uint64_t v) {
@@ -174,10 +193,13 @@ static size_t log2_sz(size_t n) {
     return l;
 }
 
-orc_cell *orc_cell_new(size_t n, size_t entry, uint64_t seed) {
-    if (n < 2 || (n & (n - 1)) || entry == 0) return NULL;
+orc_cell *orc_cell_new(size_t n, size_t entry, uint64_t seed) { return orc_cell_new_gadget(n, entry, seed, 0); }
+
+orc_cell *orc_cell_new_gadget(size_t n, size_t entry, uint64_t seed, size_t dropped) {
+    if (n < 2 || (n & (n - 1)) || entry == 0 || dropped > 1) return NULL;
     orc_cell *c = (orc_cell *)calloc(1, sizeof(orc_cell));
     c->n = n;
+    c->dropped = dropped;
     c->entry = entry;
     c->gamma = (entry + 1) / 2;
     c->records = 3 * n;
@@ -198,7 +220,7 @@ orc_cell *orc_cell_new(size_t n, size_t entry, uint64_t seed) {
     uint8_t v_seed[32];
     orc_chacha_fill_bytes(&c->rng, c->w_seed, 32);
     orc_chacha_fill_bytes(&c->rng, v_seed, 32);
-    if (orc_client_packing_keys(n, c->gamma, c->s, c->w_seed, &c->g, c->y_body)) {
+    if (orc_client_packing_keys_gadget(n, c->gamma, dropped, c->s, c->w_seed, &c->g, c->y_body)) {
         free(scratch);
         orc_cell_free(c);
         return NULL;
@@ -221,7 +243,7 @@ void orc_cell_free(orc_cell *c) {
 
 void orc_cell_info(const orc_cell *c, uint8_t w_seed[32], uint64_t *y_body, uint8_t *db, uint64_t *secret) {
     if (w_seed) memcpy(w_seed, c->w_seed, 32);
-    if (y_body) memcpy(y_body, c->y_body, 3 * c->n * 8);
+    if (y_body) memcpy(y_body, c->y_body, (3 - c->dropped) * c->n * 8);
     if (db) memcpy(db, c->db, c->records * c->entry);
     if (secret) memcpy(secret, c->s, c->n * 8);
 }
@@ -241,25 +263,125 @@ void orc_cell_columns(const orc_cell *c, uint32_t shard, uint64_t *columns) {
         }
 }
 
-/* next_query: the row seed from the cell's rng, a = expand(seed), e from the sampler,
- * b = -a*s + e + delta * X^(-local) */
-void orc_cell_query(orc_cell *c, uint64_t index, uint32_t *shard, uint8_t seed[32], uint64_t *b) {
-    size_t n = c->n;
-    *shard = (uint32_t)(index / n);
-    size_t local = (size_t)(index % n);
-    orc_chacha_fill_bytes(&c->rng, seed, 32);
+/* SeededRgswCiphertext::encrypt_with_rng at a one-row gadget: the row seed from rng,
+ * a = expand(seed), e from the sampler, b = -a*s + e + delta * X^(-local) */
+void orc_seeded_row(size_t n, const uint64_t *s, size_t local, orc_chacha *rng, orc_gauss *g, uint8_t seed[32],
+                    uint64_t *b) {
+    const uint64_t q = ORC_Q, delta = ORC_Q / 65537;
+    orc_ntt ntt;
+    if (orc_ntt_init(&ntt, n, q)) return;
+    orc_chacha_fill_bytes(rng, seed, 32);
     uint64_t *a = (uint64_t *)malloc(n * 8), *as = (uint64_t *)malloc(n * 8);
-    orc_expand_seed(seed, n, c->q, a);
-    negacyclic_mul(&c->ntt, a, c->s, as);
+    orc_expand_seed(seed, n, q, a);
+    negacyclic_mul(&ntt, a, s, as);
     for (size_t i = 0; i < n; i++) {
-        uint64_t e = from_signed(orc_gauss_sample(&c->g), c->q);
-        b[i] = (uint64_t)(((u128)(c->q - as[i]) % c->q + e) % c->q);
+        uint64_t e = from_signed(orc_gauss_sample(g), q);
+        b[i] = (uint64_t)(((u128)(q - as[i]) % q + e) % q);
     }
     /* X^(-local): 1 at 0 when local = 0, else -1 at n - local */
-    if (local == 0) b[0] = (uint64_t)(((u128)b[0] + c->delta) % c->q);
-    else b[n - local] = (uint64_t)(((u128)b[n - local] + (c->q - c->delta)) % c->q);
+    local %= n;
+    if (local == 0) b[0] = (uint64_t)(((u128)b[0] + delta) % q);
+    else b[n - local] = (uint64_t)(((u128)b[n - local] + (q - delta)) % q);
     free(a);
     free(as);
+    orc_ntt_free(&ntt);
 }
 
+/* next_query: the cell's stream at record index */
+void orc_cell_query(orc_cell *c, uint64_t index, uint32_t *shard, uint8_t seed[32], uint64_t *b) {
+    *shard = (uint32_t)(index / c->n);
+    orc_seeded_row(c->n, c->s, (size_t)(index % c->n), &c->rng, &c->g, seed, b);
+}
+
+/* heap handles so that a test can replay a draw order of its own */
+orc_gauss *orc_gauss_new(double sigma, uint64_t seed) {
+    orc_gauss *g = (orc_gauss *)calloc(1, sizeof(orc_gauss));
+    orc_gauss_with_seed(g, sigma, seed);
+    return g;
+}
+void orc_gauss_delete(orc_gauss *g) {
+    if (!g) return;
+    orc_gauss_free(g);
+    free(g);
+}
+orc_chacha *orc_chacha_new_u64(uint64_t seed) {
+    orc_chacha *r = (orc_chacha *)calloc(1, sizeof(orc_chacha));
+    orc_chacha_seed_from_u64(r, seed);
+    return r;
+}
+void orc_chacha_delete(orc_chacha *r) { free(r); }
+
 size_t orc_cell_gamma(const orc_cell *c) { return c->gamma; }
+
+/* ---------------- PIR.Extract: the client's decode of a packed response ---------------- */
+
+/* Coefficients 0..count-1 of the phase a*s + b mod (X^n + 1, q), where b holds count
+ * coefficients and is zero above them (a packed response keeps only its b prefix). s is in
+ * [0, q). With s centred below 2^32 in magnitude the negacyclic sums run exactly in i128;
+ * otherwise each product is reduced on its own. */
+void orc_phase(size_t n, uint64_t q, const uint64_t *s, const uint64_t *a, const uint64_t *b, size_t count,
+               uint64_t *out) {
+    int small = 1;
+    int64_t *sc = (int64_t *)malloc(n * sizeof(int64_t));
+    for (size_t i = 0; i < n; i++) {
+        uint64_t v = s[i] % q;
+        int64_t c = v > q / 2 ? -(int64_t)(q - v) : (int64_t)v;
+        if (c >= (1ll << 32) || c <= -(1ll << 32)) small = 0;
+        sc[i] = c;
+    }
+    for (size_t i = 0; i < count && i < n; i++) {
+        uint64_t r;
+        if (small) {
+            __int128 acc = 0;
+            for (size_t j = 0; j <= i; j++) acc += (__int128)a[j] * sc[i - j];
+            for (size_t j = i + 1; j < n; j++) acc -= (__int128)a[j] * sc[n + i - j];
+            __int128 m = acc % (__int128)q;
+            if (m < 0) m += q;
+            r = (uint64_t)m;
+        } else {
+            uint64_t pos = 0, neg = 0;
+            for (size_t j = 0; j <= i; j++) pos = (uint64_t)(((u128)pos + (u128)a[j] * (s[i - j] % q)) % q);
+            for (size_t j = i + 1; j < n; j++) neg = (uint64_t)(((u128)neg + (u128)a[j] * (s[n + i - j] % q)) % q);
+            r = (pos + q - neg) % q;
+        }
+        out[i] = (uint64_t)(((u128)r + b[i] % q) % q);
+    }
+    free(sc);
+}
+
+/* PlaintextRounding::round: floor((v + floor(delta / 2)) / delta) mod p. The reference's
+ * reciprocal estimate plus one correction is this exact quotient. */
+uint64_t orc_round_plaintext(uint64_t v, uint64_t delta, uint64_t p) {
+    u128 num = (u128)v + delta / 2;
+    return (uint64_t)(num / delta) % p;
+}
+
+/* RlweCiphertext::decrypt restricted to coefficients 0..count-1. */
+void orc_decrypt(size_t n, uint64_t q, const uint64_t *s, const uint64_t *a, const uint64_t *b, size_t count,
+                 uint64_t delta, uint64_t p, uint64_t *out) {
+    orc_phase(n, q, s, a, b, count, out);
+    for (size_t i = 0; i < count && i < n; i++) out[i] = orc_round_plaintext(out[i], delta, p);
+}
+
+/* mod_switch_secret_key: each coefficient's centred lift mod q_old (negative above
+ * floor(q_old / 2)), reduced mod q_new. */
+void orc_switch_secret(size_t n, const uint64_t *s, uint64_t q_old, uint64_t q_new, uint64_t *out) {
+    uint64_t half = q_old / 2;
+    for (size_t i = 0; i < n; i++) {
+        uint64_t c = s[i];
+        int negative = c > half;
+        uint64_t mag = negative ? q_old - c : c;
+        uint64_t red = mag % q_new;
+        out[i] = negative ? (red ? q_new - red : 0) : red;
+    }
+}
+
+/* reconstruct_entry: column j fills bytes 2j (low) and 2j+1 (high) while they fit. */
+void orc_reconstruct_entry(const uint64_t *values, size_t count, size_t entry, uint8_t *out) {
+    memset(out, 0, entry);
+    for (size_t j = 0; j < count; j++) {
+        size_t off = 2 * j;
+        if (off < entry) out[off] = (uint8_t)(values[j] & 0xff);
+        if (off + 1 < entry) out[off + 1] = (uint8_t)((values[j] >> 8) & 0xff);
+    }
+}

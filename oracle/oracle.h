@@ -165,12 +165,29 @@ int orc_client_packing_keys(size_t n, size_t gamma, const uint64_t *s, const uin
                             uint64_t *y_body);
 size_t orc_packing_keys_bytes(size_t n, const uint64_t *y_body, size_t rows, size_t gamma, uint8_t *out);
 int orc_client_key_kat(size_t n, size_t gamma, uint64_t seed, uint64_t *y_body, int64_t *next);
+int orc_client_packing_keys_gadget(size_t n, size_t gamma, size_t dropped, const uint64_t *s,
+                                   const uint8_t w_seed[32], orc_gauss *g, uint64_t *y_body);
 orc_cell *orc_cell_new(size_t n, size_t entry, uint64_t seed);
+orc_cell *orc_cell_new_gadget(size_t n, size_t entry, uint64_t seed, size_t dropped);
 void orc_cell_free(orc_cell *c);
 void orc_cell_info(const orc_cell *c, uint8_t w_seed[32], uint64_t *y_body, uint8_t *db, uint64_t *secret);
 void orc_cell_columns(const orc_cell *c, uint32_t shard, uint64_t *columns);
 void orc_cell_query(orc_cell *c, uint64_t index, uint32_t *shard, uint8_t seed[32], uint64_t *b);
 size_t orc_cell_gamma(const orc_cell *c);
+void orc_seeded_row(size_t n, const uint64_t *s, size_t local, orc_chacha *rng, orc_gauss *g, uint8_t seed[32],
+                    uint64_t *b);
+orc_gauss *orc_gauss_new(double sigma, uint64_t seed);
+void orc_gauss_delete(orc_gauss *g);
+orc_chacha *orc_chacha_new_u64(uint64_t seed);
+void orc_chacha_delete(orc_chacha *r);
+/* the client's decode (PIR.Extract): phase a*s + b, plaintext rounding, secret-key switch */
+void orc_phase(size_t n, uint64_t q, const uint64_t *s, const uint64_t *a, const uint64_t *b, size_t count,
+               uint64_t *out);
+uint64_t orc_round_plaintext(uint64_t v, uint64_t delta, uint64_t p);
+void orc_decrypt(size_t n, uint64_t q, const uint64_t *s, const uint64_t *a, const uint64_t *b, size_t count,
+                 uint64_t delta, uint64_t p, uint64_t *out);
+void orc_switch_secret(size_t n, const uint64_t *s, uint64_t q_old, uint64_t q_new, uint64_t *out);
+void orc_reconstruct_entry(const uint64_t *values, size_t count, size_t entry, uint8_t *out);
 
 /* ---------------- special modulus (src/special_modulus, special.c) ---------------- */
 typedef struct { uint32_t q, neg_qinv, r2; } sm_prime;

@@ -424,6 +424,27 @@ _lib.orc_packing_keys_bytes.argtypes = [ctypes.c_size_t, _u64p, ctypes.c_size_t,
 _lib.orc_packing_keys_bytes.restype = ctypes.c_size_t
 _lib.orc_cell_new.argtypes = [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_uint64]
 _lib.orc_cell_new.restype = ctypes.c_void_p
+_lib.orc_cell_new_gadget.argtypes = [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_size_t]
+_lib.orc_cell_new_gadget.restype = ctypes.c_void_p
+_lib.orc_client_packing_keys_gadget.argtypes = [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _u64p, _u8p,
+                                                ctypes.c_void_p, _u64p]
+_lib.orc_seeded_row.argtypes = [ctypes.c_size_t, _u64p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, _u8p,
+                                _u64p]
+_lib.orc_gauss_new.argtypes = [ctypes.c_double, ctypes.c_uint64]
+_lib.orc_gauss_new.restype = ctypes.c_void_p
+_lib.orc_gauss_delete.argtypes = [ctypes.c_void_p]
+_lib.orc_sample_poly.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64, _u64p]
+_lib.orc_chacha_new_u64.argtypes = [ctypes.c_uint64]
+_lib.orc_chacha_new_u64.restype = ctypes.c_void_p
+_lib.orc_chacha_delete.argtypes = [ctypes.c_void_p]
+_lib.orc_chacha_fill_bytes.argtypes = [ctypes.c_void_p, _u8p, ctypes.c_size_t]
+_lib.orc_phase.argtypes = [ctypes.c_size_t, ctypes.c_uint64, _u64p, _u64p, _u64p, ctypes.c_size_t, _u64p]
+_lib.orc_round_plaintext.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
+_lib.orc_round_plaintext.restype = ctypes.c_uint64
+_lib.orc_decrypt.argtypes = [ctypes.c_size_t, ctypes.c_uint64, _u64p, _u64p, _u64p, ctypes.c_size_t,
+                             ctypes.c_uint64, ctypes.c_uint64, _u64p]
+_lib.orc_switch_secret.argtypes = [ctypes.c_size_t, _u64p, ctypes.c_uint64, ctypes.c_uint64, _u64p]
+_lib.orc_reconstruct_entry.argtypes = [_u64p, ctypes.c_size_t, ctypes.c_size_t, _u8p]
 _lib.orc_cell_free.argtypes = [ctypes.c_void_p]
 _lib.orc_cell_info.argtypes = [ctypes.c_void_p, _u8p, _u64p, _u8p, _u64p]
 _lib.orc_cell_columns.argtypes = [ctypes.c_void_p, ctypes.c_uint32, _u64p]
@@ -461,16 +482,19 @@ def apply_automorphism(poly, t, q=Q):
 
 class Cell:
     """A packing-key KAT cell (three shards, one client session, a reproducible query
-    stream), built in the reference fixture's draw order."""
+    stream), built in the reference fixture's draw order (tests/packing_key_fixture/mod.rs:89-113).
+    approximate=True builds the cell under d2048_approximate_gadget (two key rows)."""
 
-    def __init__(self, n, entry, seed):
-        self._c = _lib.orc_cell_new(n, entry, seed)
+    def __init__(self, n, entry, seed, approximate=False):
+        self._c = _lib.orc_cell_new_gadget(n, entry, seed, 1 if approximate else 0)
         if not self._c:
             raise ValueError(f"no cell at ring {n}, entry {entry}")
         self.n, self.entry, self.seed = n, entry, seed
+        self.approximate = approximate
+        self.rows = 2 if approximate else 3
         self.gamma = _lib.orc_cell_gamma(self._c)
         w = (ctypes.c_uint8 * 32)()
-        self.y_body = np.zeros((3, n), np.uint64)
+        self.y_body = np.zeros((self.rows, n), np.uint64)
         self.db = np.zeros(3 * n * entry, np.uint8)
         self.secret = np.zeros(n, np.uint64)
         _lib.orc_cell_info(self._c, ctypes.cast(w, _u8p), _p64(self.y_body),
@@ -497,6 +521,249 @@ class Cell:
         b = np.zeros(self.n, np.uint64)
         _lib.orc_cell_query(self._c, index, ctypes.byref(shard), ctypes.cast(seed, _u8p), _p64(b))
         return shard.value, bytes(seed), b
+
+
+# ---------------- the client's decode (PIR.Extract) ----------------
+
+P = 65537                               # InspireParams::d2048().p, src/params.rs:363
+SIGMA = 6.4
+MOD_SWITCH_TARGET_45BIT = 35_184_372_060_161   # src/pir/mod_switch.rs:33
+IMPLEMENTED_TARGETS = (MOD_SWITCH_TARGET_45BIT, Q36)  # src/pir/mod_switch.rs:53
+
+
+class Gauss:
+    """GaussianSampler::with_seed(sigma, seed) (src/math/gaussian.rs)."""
+
+    def __init__(self, seed, sigma=SIGMA):
+        self._g = _lib.orc_gauss_new(sigma, seed)
+
+    def __del__(self):
+        if getattr(self, "_g", None):
+            _lib.orc_gauss_delete(self._g)
+            self._g = None
+
+    def poly(self, n, q=Q):
+        """Poly::sample_gaussian_moduli: n draws, each lifted into [0, q)."""
+        out = np.zeros(n, np.uint64)
+        _lib.orc_sample_poly(self._g, n, q, _p64(out))
+        return out
+
+
+class ChaCha:
+    """ChaCha20Rng::seed_from_u64(seed)."""
+
+    def __init__(self, seed):
+        self._r = _lib.orc_chacha_new_u64(seed)
+
+    def __del__(self):
+        if getattr(self, "_r", None):
+            _lib.orc_chacha_delete(self._r)
+            self._r = None
+
+    def fill(self, count):
+        buf = (ctypes.c_uint8 * count)()
+        _lib.orc_chacha_fill_bytes(self._r, ctypes.cast(buf, _u8p), count)
+        return bytes(buf)
+
+
+def client_packing_keys(n, gamma, secret, w_seed, gauss, approximate=False):
+    """ClientPackingKeys::generate_once (src/inspiring/inspiring2.rs:794-831): the key rows
+    y_body, drawing one error polynomial per kept row from gauss."""
+    rows = 2 if approximate else 3
+    y = np.zeros((rows, n), np.uint64)
+    s_ = np.ascontiguousarray(secret, dtype=np.uint64)
+    wp, _wbuf = _seed(w_seed)
+    if _lib.orc_client_packing_keys_gadget(n, gamma, 1 if approximate else 0, _p64(s_), wp, gauss._g,
+                                           _p64(y)) != 0:
+        raise ValueError(f"width {gamma} is not legal at ring {n}")
+    return y
+
+
+def seeded_row(secret, local, rng, gauss):
+    """SeededRgswCiphertext::encrypt_with_rng of delta * X^(-local) at a one-row gadget
+    (src/rgsw/types.rs:345-386): (row seed, b)."""
+    s_ = np.ascontiguousarray(secret, dtype=np.uint64)
+    n = s_.size
+    seed = (ctypes.c_uint8 * 32)()
+    b = np.zeros(n, np.uint64)
+    _lib.orc_seeded_row(n, _p64(s_), local, rng._r, gauss._g, ctypes.cast(seed, _u8p), _p64(b))
+    return bytes(seed), b
+
+
+def phase(secret, a, b, count, q=Q):
+    """Coefficients 0..count-1 of a*s + b mod (X^n + 1, q); b may hold only count coefficients."""
+    s_ = np.ascontiguousarray(secret, dtype=np.uint64)
+    a_ = np.ascontiguousarray(a, dtype=np.uint64)
+    n = a_.size
+    bb = np.zeros(n, np.uint64)
+    bb[:min(count, len(b))] = np.asarray(b, dtype=np.uint64)[:count]
+    out = np.zeros(n, np.uint64)
+    _lib.orc_phase(n, q, _p64(s_), _p64(a_), _p64(bb), count, _p64(out))
+    return out[:count]
+
+
+def round_plaintext(value, delta, p=P):
+    """PlaintextRounding::round (src/rlwe/enc.rs:35-43)."""
+    return _lib.orc_round_plaintext(value, delta, p)
+
+
+def switch_secret(secret, q_new, q_old=Q):
+    """mod_switch_secret_key (src/pir/mod_switch.rs:221-252)."""
+    s_ = np.ascontiguousarray(secret, dtype=np.uint64)
+    out = np.zeros_like(s_)
+    _lib.orc_switch_secret(s_.size, _p64(s_), q_old, q_new, _p64(out))
+    return out
+
+
+def reconstruct_entry(values, entry):
+    """reconstruct_entry (src/pir/encode_db.rs:161-176)."""
+    v = np.ascontiguousarray(values, dtype=np.uint64)
+    out = (ctypes.c_uint8 * entry)()
+    _lib.orc_reconstruct_entry(_p64(v), v.size, entry, ctypes.cast(out, _u8p))
+    return bytes(out)
+
+
+def check_mod_switch_noise_budget(n, target, q=Q, p=P, sigma=SIGMA):
+    """check_mod_switch_noise_budget (src/pir/mod_switch.rs:111-144), with the target checks of
+    ModSwitchParams::new (:67-98). Raises ValueError on a refusal."""
+    if target == 0:
+        raise ValueError("mod-switch target_modulus must be positive")
+    if target > q:
+        raise ValueError(f"mod-switch target_modulus {target} exceeds source modulus {q}; "
+                         "widening is not a mod-switch")
+    if target < p:
+        raise ValueError(f"mod-switch target_modulus {target} < plaintext modulus p={p}; "
+                         "decryption impossible")
+    if target % (2 * n) != 1:
+        raise ValueError(f"mod-switch target_modulus {target} is not NTT-friendly: must satisfy "
+                         f"q' == 1 (mod 2*d) where d={n}")
+    s_max = 7.0 * sigma
+    round_noise = 0.5 + n * 0.5 * s_max
+    scaled_old = (target / q) * (q / (4.0 * p))
+    total = round_noise + scaled_old + float(target % p)
+    budget = target / (2.0 * p)
+    if total >= budget:
+        raise ValueError(f"mod-switch noise-budget violation: total_noise={total:.0f} >= budget={budget:.0f}")
+    return budget / total
+
+
+def extract_inspiring(secret, a, b_prefix, entry, retained=None, modulus=Q, p=P):
+    """extract_inspiring (src/pir/extract.rs:229-263) and, for a response on another modulus,
+    extract_inspiring_mod_switched (src/pir/mod_switch.rs:573-639): the entry bytes the
+    response decodes to under the client's secret (in [0, Q))."""
+    a_ = np.ascontiguousarray(a, dtype=np.uint64)
+    n = a_.size
+    cols = (entry + 1) // 2
+    if retained is not None and retained != cols:
+        raise ValueError(f"extract_inspiring: packed response retains {retained} coefficients, "
+                         f"{cols} required")
+    if modulus == Q:
+        s_, delta = np.asarray(secret, np.uint64), Q // p
+    else:
+        if modulus not in IMPLEMENTED_TARGETS:
+            raise ValueError(f"response modulus {modulus} is not an implemented mod-switch target")
+        check_mod_switch_noise_budget(n, modulus, p=p)
+        s_, delta = switch_secret(secret, modulus), modulus // p
+    values = [round_plaintext(int(v), delta, p) for v in phase(s_, a_, b_prefix, cols, modulus)]
+    return reconstruct_entry(np.array(values, np.uint64), entry)
+
+
+def served_post_switch_error(secret, a, b_prefix, modulus, record, p=P):
+    """served_post_switch_error (src/pir/mod_switch.rs:1027-1059): the largest distance, over the
+    response's retained coefficients, between the phase under the switched secret and
+    m * floor(q'/p) for the record's column value m."""
+    rec = np.frombuffer(bytes(record), np.uint8)
+    cols = len(b_prefix)
+    msg = rec[0:2 * cols:2].astype(np.uint64) | (rec[1:2 * cols:2].astype(np.uint64) << 8)
+    s_ = secret if modulus == Q else switch_secret(secret, modulus)
+    noisy = phase(s_, a, b_prefix, cols, modulus)
+    dp = modulus // p
+    worst = 0
+    for i in range(cols):
+        d = abs(int(noisy[i]) - int(msg[i]) * dp % modulus)
+        worst = max(worst, min(d, modulus - d))
+    return worst
+
+
+def post_switch_error_kat_inputs():
+    """The client side of production_post_switch_error_kat_at_45_36_and_33_bits
+    (src/pir/mod_switch.rs:1070-1100), in its draw order: setup_with_rng at sampler seed 0x5002
+    and setup rng 0x5002_0001 (secret, 11 tree-packing key matrices of 3 error rows, w_seed,
+    v_seed), query_seeded's own row error and its inline packing keys, then the row request
+    re-encrypted under sampler 0x5002_0002 and rng 0x5002_0003 for record 17 of a 2048 x 512-byte
+    database whose byte k is (37 k + 11) mod 256."""
+    n, entry, gamma, target = 2048, 512, 256, 17
+    g = Gauss(0x5002)
+    secret = g.poly(n)
+    for _ in range(11 * 3):
+        g.poly(n)
+    rng = ChaCha(0x5002_0001)
+    w_seed = rng.fill(32)
+    rng.fill(32)
+    g.poly(n)
+    y_body = client_packing_keys(n, gamma, secret, w_seed, g)
+    seed, b = seeded_row(secret, target, ChaCha(0x5002_0003), Gauss(0x5002_0002))
+    db = ((np.arange(n * entry, dtype=np.int64) * 37 + 11) % 256).astype(np.uint8)
+    rec = db.reshape(n, entry)
+    columns = (rec[:, 0::2].astype(np.uint64) | (rec[:, 1::2].astype(np.uint64) << 8)).T.copy()
+    return {"n": n, "entry": entry, "gamma": gamma, "target": target, "secret": secret, "w_seed": w_seed,
+            "y_body": y_body, "seed": seed, "b": b, "db": db, "columns": columns,
+            "record": rec[target].tobytes()}
+
+
+# (target modulus, recorded post-switch error) of src/pir/mod_switch.rs:1102-1107
+POST_SWITCH_ERROR_KAT = ((MOD_SWITCH_TARGET_45BIT, 317_711), (Q36, 652), (68_719_464_449, 53_503),
+                         (8_589_905_921, 36_836))
+
+
+def parse_response(data):
+    """The packed InspiRING ServerResponse bincode (respond.rs:111-176, poly.rs:166-225,
+    286-310): {"a", "b_prefix", "modulus", "retained"}. Raises ValueError on another layout."""
+    import struct
+    at = 0
+
+    def take(fmt):
+        nonlocal at
+        if at + struct.calcsize(fmt) > len(data):
+            raise ValueError("truncated response")
+        v = struct.unpack_from("<" + fmt, data, at)[0]
+        at += struct.calcsize(fmt)
+        return v
+
+    def tight(nbytes, count, w):
+        nonlocal at
+        raw = np.frombuffer(bytes(data[at:at + nbytes]), np.uint8)
+        at += nbytes
+        bits = np.unpackbits(raw, bitorder="little")[:count * w].reshape(count, w).astype(np.uint64)
+        return (bits << np.arange(w, dtype=np.uint64)).sum(axis=1, dtype=np.uint64)
+
+    if take("I") != 1:
+        raise ValueError("not a packed response")
+    a_len = take("Q")
+    a_at = at
+    at += a_len
+    if take("Q") != 1:
+        raise ValueError("a: one limb expected")
+    modulus = take("Q")
+    if take("Q") != modulus:
+        raise ValueError("a: q differs from its limb")
+    n = take("Q")
+    if take("Q") != 0 or take("B") != 0:
+        raise ValueError("a: crt inverse or NTT flag set")
+    w = int(modulus - 1).bit_length()
+    b_len = take("Q")
+    b_at = at
+    at += b_len
+    retained = take("I")
+    if take("Q") != 0 or take("B") != 1 or take("I") != 0:
+        raise ValueError("trailer is not Some(Inspiring) with no column ciphertexts")
+    if at != len(data) or a_len != (n * w + 7) // 8 or b_len != (retained * w + 7) // 8:
+        raise ValueError("length fields do not match the layout")
+    at = a_at
+    a = tight(a_len, n, w)
+    at = b_at
+    b = tight(b_len, retained, w)
+    return {"a": a, "b_prefix": b, "modulus": modulus, "retained": retained}
 
 
 def load_packing_seat_kat(path):

@@ -85,11 +85,16 @@ def test_acc128_sums_of_products(st):
     assert st.st_acc_reduce(a.ctypes.data, a.ctypes.data, 16, Q - 1) == (Q - 1 + 16 * (Q - 1) ** 2) % Q
 
 
-def test_mod_switch_matches_oracle(st):
+@pytest.mark.parametrize("qn", [oracle.Q36, oracle.MOD_SWITCH_TARGET_45BIT, 68_719_464_449, 8_589_905_921,
+                                (1 << 46) - 1])
+def test_mod_switch_matches_oracle(st, qn):
+    """Every implemented target, two other NTT primes and the largest modulus the device
+    accepts, against the oracle's u128 switch and exact big integers."""
     rng = np.random.default_rng(4)
     vals = np.array(EDGES[:6] + [int(x) for x in rng.integers(0, Q, 2000, dtype=np.uint64)], dtype=np.uint64)
-    want = oracle.mod_switch(vals)
-    assert [st.st_mod_switch(int(v), oracle.Q36) for v in vals] == [int(w) for w in want]
+    want = oracle.mod_switch(vals, oracle.Q, qn)
+    assert [int(w) for w in want] == [((int(v) * qn + Q // 2) // Q) % qn for v in vals]
+    assert [st.st_mod_switch(int(v), qn) for v in vals] == [int(w) for w in want]
 
 
 def test_mod_p61(st):
