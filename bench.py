@@ -105,17 +105,21 @@ def cpu_baseline(budget_s=10.0):
                       f"{T} threads for {dtT:.1f} s; and {n1} on 1 thread for {dt1:.1f} s"}
 
 
-def served(g, seconds, rate_qps, seed=7):
+def served(g, seconds, rate_qps, seed=7, capacity_queries=0, queries_per_cu=2):
     """Served config: d2048_approximate_gadget, 2^20 x 512 B records (512 shards, stores
     precomputed), 1024 seated client keys registered by handle, Poisson arrivals of
     batches of 1-32 queries at uniform indices answered by the continuous-batching
-    server. Inputs come from host memory (PCIe-inclusive); latency is submit to result."""
+    server. Inputs come from host memory (PCIe-inclusive); latency is submit to result.
+
+    Two runs over the same server: arrivals paced at rate_qps (the latency operating
+    point), then, when capacity_queries > 0, that many queries submitted at once as a
+    backlog, whose drain rate is the server's capacity."""
     import threading
 
     rng = np.random.default_rng(seed)
     w_seed = rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
     ctx = g.Context(g.params_d2048(approximate=True), w_seed, GAMMA)
-    ctx.set_queries_per_cu(2)
+    ctx.set_queries_per_cu(queries_per_cu)
     shards = 512
     t0 = time.perf_counter()
     for s in range(shards):
@@ -123,65 +127,86 @@ def served(g, seconds, rate_qps, seed=7):
     build = time.perf_counter() - t0
     server = g.Server(ctx, max_launch=512)
     handles = [server.register_seat(rng.integers(0, g.Q, (2, N), dtype=np.uint64)) for _ in range(1024)]
-    # schedule
-    reqs, t = [], 0.0
-    while t < seconds:
-        count = int(rng.integers(1, 33))
-        t += rng.exponential(count / rate_qps)
-        reqs.append((t, int(rng.integers(0, len(handles))), rng.integers(0, shards, count).astype(np.uint32),
-                     [rng.integers(0, 256, 32, dtype=np.uint8).tobytes() for _ in range(count)],
-                     rng.integers(0, g.Q, (count, N), dtype=np.uint64)))
+    # query bodies come from a pool: their values do not change the work
+    pool = rng.integers(0, g.Q, (64, 32, N), dtype=np.uint64)
+
+    def schedule(total_s=None, total_q=None):
+        reqs, t, q = [], 0.0, 0
+        while (total_s is not None and t < total_s) or (total_q is not None and q < total_q):
+            count = int(rng.integers(1, 33))
+            t += rng.exponential(count / rate_qps)
+            q += count
+            reqs.append((t, int(rng.integers(0, len(handles))), rng.integers(0, shards, count).astype(np.uint32),
+                         [rng.integers(0, 256, 32, dtype=np.uint8).tobytes() for _ in range(count)],
+                         pool[int(rng.integers(0, len(pool)))][:count]))
+        return reqs
+
+    def run(reqs, paced):
+        lats, done_q = [], [0]
+        lock = threading.Lock()
+        tickets = []
+
+        def waiter():
+            while True:
+                with lock:
+                    if not tickets:
+                        if waiter.stop:
+                            return
+                        tk = None
+                    else:
+                        tk = tickets.pop(0)
+                if tk is None:
+                    time.sleep(0.0005)
+                    continue
+                a, b, lat = server.wait(tk)
+                with lock:
+                    lats.append(lat)
+                    done_q[0] += a.shape[0]
+
+        waiter.stop = False
+        ws_threads = [threading.Thread(target=waiter) for _ in range(4)]
+        for th in ws_threads:
+            th.start()
+        launches0 = server.stats()["launches"]
+        start = time.perf_counter()
+        for (at, h, ids, seeds, bs) in reqs:
+            if paced:
+                delay = start + at - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+            tk = server.submit(handles[h], ids, seeds, bs, switch_modulus=Q36)
+            with lock:
+                tickets.append(tk)
+        waiter.stop = True
+        for th in ws_threads:
+            th.join()
+        elapsed = time.perf_counter() - start
+        return done_q[0], elapsed, np.array(lats), server.stats()["launches"] - launches0
+
+    reqs = schedule(total_s=seconds)
     # warm the server
     server.wait(server.submit(handles[0], reqs[0][2], reqs[0][3], reqs[0][4], switch_modulus=Q36))
-    lats, done_q = [], [0]
-    lock = threading.Lock()
-    tickets = []
-
-    def waiter():
-        while True:
-            with lock:
-                if not tickets:
-                    if waiter.stop:
-                        return
-                    tk = None
-                else:
-                    tk = tickets.pop(0)
-            if tk is None:
-                time.sleep(0.0005)
-                continue
-            a, b, lat = server.wait(tk)
-            with lock:
-                lats.append(lat)
-                done_q[0] += a.shape[0]
-
-    waiter.stop = False
-    ws_threads = [threading.Thread(target=waiter) for _ in range(4)]
-    for th in ws_threads:
-        th.start()
-    start = time.perf_counter()
-    for (at, h, ids, seeds, bs) in reqs:
-        delay = start + at - time.perf_counter()
-        if delay > 0:
-            time.sleep(delay)
-        tk = server.submit(handles[h], ids, seeds, bs, switch_modulus=Q36)
-        with lock:
-            tickets.append(tk)
-    waiter.stop = True
-    for th in ws_threads:
-        th.join()
-    elapsed = time.perf_counter() - start
+    done, elapsed, lats, launches = run(reqs, paced=True)
+    capacity = None
+    if capacity_queries > 0:
+        creqs = schedule(total_q=capacity_queries)
+        cdone, celapsed, clats, claunches = run(creqs, paced=False)
+        capacity = {"value": round(cdone / celapsed, 1), "unit": "queries/s", "queries": cdone,
+                    "requests": len(creqs), "launches": claunches, "seconds": round(celapsed, 3),
+                    "mean_queries_per_launch": round(cdone / max(claunches, 1), 1),
+                    "workload": "the same request mix submitted at once as a backlog; value = drain rate"}
     stats = server.stats()
     server.close()
     ctx.close()
-    lats = np.array(lats)
-    return {"offered_qps": rate_qps, "value": round(done_q[0] / elapsed, 1), "unit": "queries/s",
+    return {"offered_qps": round(rate_qps, 1), "value": round(done / elapsed, 1), "unit": "queries/s",
             "latency_ms": {"p50": round(float(np.percentile(lats, 50)), 3),
                            "p99": round(float(np.percentile(lats, 99)), 3)},
-            "requests": len(lats), "launches": stats["launches"],
+            "requests": len(lats), "launches": launches,
             "seats_resident_bytes": stats["resident_bytes"], "store_build_s": round(build, 2),
+            "capacity": capacity,
             "workload": "d2048_approximate_gadget, 2^20 x 512 B (512 shards), 1024 seated keys, Poisson arrivals "
-                        "of 1-32-query batches, host inputs (PCIe-inclusive), continuous batching <= 512 queries "
-                        "per launch, 36-bit switch"}
+                        "of 1-32-query batches at 70% of the batched rate, host inputs (PCIe-inclusive), "
+                        "continuous batching <= 512 queries per launch, 36-bit switch"}
 
 
 SM_N, SM_GAMMA, SM_CELL, SM_RECORD = 2048, 512, 256, 246
@@ -598,7 +623,9 @@ def main():
     serve = None
     if not args.no_served:
         log("bench: served config")
-        serve = served(g, args.served_seconds, rate_qps=0.7 * qps)
+        serve = served(g, args.served_seconds, rate_qps=0.7 * qps,
+                       capacity_queries=int(max(args.served_seconds, 1.0) * qps),
+                       queries_per_cu=tuned["queries_per_cu"] if tuned else args.queries_per_cu)
 
     cpu = None
     if not args.no_cpu:

@@ -55,13 +55,17 @@ struct Slot {
     uint64_t *h_a = nullptr, *h_bp = nullptr;
     int32_t *h_status = nullptr;
     DevBuf d_ids, d_seeds, d_b, d_seats, d_a, d_bp, scan_part, status;
-    hipEvent_t done_ev = nullptr;
+    // inputs copied (copy-in stream), kernels done (compute stream), outputs copied (copy-out
+    // stream): the last one completes the slot
+    hipEvent_t in_ev = nullptr, kern_ev = nullptr, done_ev = nullptr;
     std::vector<Request> reqs;
     size_t count = 0;
     bool busy = false;
     ~Slot() {
         (void)hipHostFree(h_ids); (void)hipHostFree(h_seeds); (void)hipHostFree(h_b);
         (void)hipHostFree(h_seats); (void)hipHostFree(h_a); (void)hipHostFree(h_bp); (void)hipHostFree(h_status);
+        if (in_ev) (void)hipEventDestroy(in_ev);
+        if (kern_ev) (void)hipEventDestroy(kern_ev);
         if (done_ev) (void)hipEventDestroy(done_ev);
     }
 };
@@ -73,7 +77,10 @@ struct inspire_server {
     size_t max_launch = 0;
     size_t hbm_budget = 0;     // 0 = free device memory only
     size_t resident = 0;       // seat bytes admitted
-    hipStream_t stream = nullptr;
+    // Kernels run on `stream`; a slot's inputs go up on `copy_in` and its outputs come back
+    // on `copy_out`, so the copies of one launch overlap the kernels of its neighbours. All
+    // three are non-blocking streams at the default priority.
+    hipStream_t stream = nullptr, copy_in = nullptr, copy_out = nullptr;
     Slot slots[2];
 
     std::mutex mu;
@@ -110,6 +117,8 @@ int alloc_slot(inspire_server *sv, Slot &s, char *err, size_t err_cap) {
     HIP_TRY(s.d_bp.alloc(c * g * 8));
     HIP_TRY(s.scan_part.alloc(c * sv->ctx->ks() * g * 4 * 8));
     HIP_TRY(s.status.alloc(c * 4));
+    HIP_TRY(hipEventCreateWithFlags(&s.in_ev, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&s.kern_ev, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&s.done_ev, hipEventDisableTiming));
     s.cap = c;
     return INSPIRE_OK;
@@ -136,10 +145,14 @@ int inspire_server::launch(Slot &s, char *err, size_t err_cap) {
         if (rc) return rc;
         dev = ctx->dev();  // the shard table and launch shape as of this launch
     }
-    HIP_TRY(hipMemcpyAsync(s.d_ids.p, s.h_ids, q * 4, hipMemcpyHostToDevice, stream));
-    HIP_TRY(hipMemcpyAsync(s.d_seeds.p, s.h_seeds, q * 32, hipMemcpyHostToDevice, stream));
-    HIP_TRY(hipMemcpyAsync(s.d_b.p, s.h_b, q * n * 8, hipMemcpyHostToDevice, stream));
-    HIP_TRY(hipMemcpyAsync(s.d_seats.p, s.h_seats, q * sizeof(void *), hipMemcpyHostToDevice, stream));
+    // The slot's previous launch has completed (its done_ev, after its output copies), so its
+    // buffers are free; the copies may run while the other slot's kernels do.
+    HIP_TRY(hipMemcpyAsync(s.d_ids.p, s.h_ids, q * 4, hipMemcpyHostToDevice, copy_in));
+    HIP_TRY(hipMemcpyAsync(s.d_seeds.p, s.h_seeds, q * 32, hipMemcpyHostToDevice, copy_in));
+    HIP_TRY(hipMemcpyAsync(s.d_b.p, s.h_b, q * n * 8, hipMemcpyHostToDevice, copy_in));
+    HIP_TRY(hipMemcpyAsync(s.d_seats.p, s.h_seats, q * sizeof(void *), hipMemcpyHostToDevice, copy_in));
+    HIP_TRY(hipEventRecord(s.in_ev, copy_in));
+    HIP_TRY(hipStreamWaitEvent(stream, s.in_ev, 0));
     BatchDev bd;
     bd.count = (int)q;
     bd.shard_ids = s.d_ids.as<uint32_t>();
@@ -155,10 +168,12 @@ int inspire_server::launch(Slot &s, char *err, size_t err_cap) {
     bd.status = s.status.as<int32_t>();
     bd.switch_modulus = s.reqs.front().switch_modulus;
     HIP_TRY(launch_respond(dev, bd, stream));
-    HIP_TRY(hipMemcpyAsync(s.h_a, s.d_a.p, q * n * 8, hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipMemcpyAsync(s.h_bp, s.d_bp.p, q * g * 8, hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipMemcpyAsync(s.h_status, s.status.p, q * 4, hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipEventRecord(s.done_ev, stream));
+    HIP_TRY(hipEventRecord(s.kern_ev, stream));
+    HIP_TRY(hipStreamWaitEvent(copy_out, s.kern_ev, 0));
+    HIP_TRY(hipMemcpyAsync(s.h_a, s.d_a.p, q * n * 8, hipMemcpyDeviceToHost, copy_out));
+    HIP_TRY(hipMemcpyAsync(s.h_bp, s.d_bp.p, q * g * 8, hipMemcpyDeviceToHost, copy_out));
+    HIP_TRY(hipMemcpyAsync(s.h_status, s.status.p, q * 4, hipMemcpyDeviceToHost, copy_out));
+    HIP_TRY(hipEventRecord(s.done_ev, copy_out));
     launches++;
     queries += q;
     return INSPIRE_OK;
@@ -232,6 +247,10 @@ void inspire_server::run() {
         char err[512] = {0};
         int rc = launch(s, err, sizeof err);
         if (rc != INSPIRE_OK) {
+            // a partly enqueued launch may still read or write the slot's buffers
+            (void)hipStreamSynchronize(copy_in);
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamSynchronize(copy_out);
             std::lock_guard<std::mutex> lk(mu);
             for (Request &r : s.reqs) {
                 Result &res = results[r.ticket];
@@ -263,16 +282,23 @@ int inspire_server_create(inspire_ctx *ctx, size_t max_launch_queries, size_t hb
     sv->ctx = ctx;
     sv->max_launch = max_launch_queries;
     sv->hbm_budget = hbm_budget_bytes;
-    hipError_t e = hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        set_err(err, err_cap, "HIP failure creating the server stream: %s", hipGetErrorString(e));
-        delete sv;
-        return INSPIRE_ERR_DEVICE;
+    auto drop_streams = [sv] {
+        for (hipStream_t st : {sv->stream, sv->copy_in, sv->copy_out})
+            if (st) (void)hipStreamDestroy(st);
+    };
+    for (hipStream_t *st : {&sv->stream, &sv->copy_in, &sv->copy_out}) {
+        hipError_t e = hipStreamCreateWithFlags(st, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            set_err(err, err_cap, "HIP failure creating the server streams: %s", hipGetErrorString(e));
+            drop_streams();
+            delete sv;
+            return INSPIRE_ERR_DEVICE;
+        }
     }
     for (Slot &s : sv->slots) {
         int rc = alloc_slot(sv, s, err, err_cap);
         if (rc) {
-            (void)hipStreamDestroy(sv->stream);
+            drop_streams();
             delete sv;
             return rc;
         }
@@ -290,8 +316,10 @@ void inspire_server_destroy(inspire_server *sv) {
     }
     sv->cv_work.notify_all();
     sv->worker.join();
-    (void)hipStreamSynchronize(sv->stream);
-    (void)hipStreamDestroy(sv->stream);
+    for (hipStream_t st : {sv->copy_in, sv->stream, sv->copy_out}) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
     delete sv;
 }
 
