@@ -55,12 +55,16 @@ struct Slot {
     uint8_t *h_seeds = nullptr;
     uint32_t *h_bodies = nullptr, *h_mask = nullptr, *h_body = nullptr;
     Buf d_jobs, d_seeds, d_bodies, d_mask, d_body;
-    hipEvent_t done_ev = nullptr;
+    // inputs copied (copy-in stream), kernel done (compute stream), outputs copied (copy-out
+    // stream): the last one completes the slot
+    hipEvent_t in_ev = nullptr, kern_ev = nullptr, done_ev = nullptr;
     std::vector<Request> reqs;
     bool busy = false;
     ~Slot() {
         (void)hipHostFree(h_jobs); (void)hipHostFree(h_seeds); (void)hipHostFree(h_bodies);
         (void)hipHostFree(h_mask); (void)hipHostFree(h_body);
+        if (in_ev) (void)hipEventDestroy(in_ev);
+        if (kern_ev) (void)hipEventDestroy(kern_ev);
         if (done_ev) (void)hipEventDestroy(done_ev);
     }
 };
@@ -71,7 +75,10 @@ struct inspire_sm_batcher {
     inspire_sm_server *srv = nullptr;
     size_t max_launch = 0, hbm_budget = 0, resident = 0;
     size_t n = 0, nb = 0;
-    hipStream_t stream = nullptr;
+    // The kernel runs on `stream`; a slot's inputs go up on `copy_in` and its outputs come
+    // back on `copy_out`, so one launch's copies overlap its neighbours' kernels. All three
+    // are non-blocking streams at the default priority.
+    hipStream_t stream = nullptr, copy_in = nullptr, copy_out = nullptr;
     Slot slots[2];
 
     std::mutex mu;
@@ -103,6 +110,8 @@ int alloc_slot(inspire_sm_batcher *b, Slot &s, char *err, size_t err_cap) {
     SM_TRY(s.d_bodies.alloc(c * n * 4));
     SM_TRY(s.d_mask.alloc(c * n * 4));
     SM_TRY(s.d_body.alloc(c * nb * 4));
+    SM_TRY(hipEventCreateWithFlags(&s.in_ev, hipEventDisableTiming));
+    SM_TRY(hipEventCreateWithFlags(&s.kern_ev, hipEventDisableTiming));
     SM_TRY(hipEventCreateWithFlags(&s.done_ev, hipEventDisableTiming));
     return INSPIRE_OK;
 }
@@ -120,15 +129,20 @@ int inspire_sm_batcher::launch(Slot &s, char *err, size_t err_cap) {
         memcpy(s.h_bodies + q * n, r.bodies.data(), c * n * 4);
         q += c;
     }
-    SM_TRY(hipMemcpyAsync(s.d_jobs.p, s.h_jobs, q * sizeof(SmJob), hipMemcpyHostToDevice, stream));
-    SM_TRY(hipMemcpyAsync(s.d_seeds.p, s.h_seeds, q * 32, hipMemcpyHostToDevice, stream));
-    SM_TRY(hipMemcpyAsync(s.d_bodies.p, s.h_bodies, q * n * 4, hipMemcpyHostToDevice, stream));
+    // the slot's previous launch has completed (its done_ev), so its buffers are free
+    SM_TRY(hipMemcpyAsync(s.d_jobs.p, s.h_jobs, q * sizeof(SmJob), hipMemcpyHostToDevice, copy_in));
+    SM_TRY(hipMemcpyAsync(s.d_seeds.p, s.h_seeds, q * 32, hipMemcpyHostToDevice, copy_in));
+    SM_TRY(hipMemcpyAsync(s.d_bodies.p, s.h_bodies, q * n * 4, hipMemcpyHostToDevice, copy_in));
+    SM_TRY(hipEventRecord(s.in_ev, copy_in));
+    SM_TRY(hipStreamWaitEvent(stream, s.in_ev, 0));
     SmBatchDev bd{(int)q, s.d_jobs.as<SmJob>(), s.d_seeds.as<uint8_t>(), s.d_bodies.as<uint32_t>(),
                   s.d_mask.as<uint32_t>(), s.d_body.as<uint32_t>()};
     SM_TRY(sm_launch_respond(srv->dev, bd, stream));
-    SM_TRY(hipMemcpyAsync(s.h_mask, s.d_mask.p, q * n * 4, hipMemcpyDeviceToHost, stream));
-    SM_TRY(hipMemcpyAsync(s.h_body, s.d_body.p, q * nb * 4, hipMemcpyDeviceToHost, stream));
-    SM_TRY(hipEventRecord(s.done_ev, stream));
+    SM_TRY(hipEventRecord(s.kern_ev, stream));
+    SM_TRY(hipStreamWaitEvent(copy_out, s.kern_ev, 0));
+    SM_TRY(hipMemcpyAsync(s.h_mask, s.d_mask.p, q * n * 4, hipMemcpyDeviceToHost, copy_out));
+    SM_TRY(hipMemcpyAsync(s.h_body, s.d_body.p, q * nb * 4, hipMemcpyDeviceToHost, copy_out));
+    SM_TRY(hipEventRecord(s.done_ev, copy_out));
     launches++;
     queries += q;
     return INSPIRE_OK;
@@ -187,6 +201,10 @@ void inspire_sm_batcher::run() {
         char err[512] = {0};
         int rc = launch(s, err, sizeof err);
         if (rc != INSPIRE_OK) {
+            // a partly enqueued launch may still read or write the slot's buffers
+            (void)hipStreamSynchronize(copy_in);
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamSynchronize(copy_out);
             std::lock_guard<std::mutex> lk(mu);
             for (Request &r : s.reqs) {
                 Result &res = results[r.ticket];
@@ -220,16 +238,23 @@ int inspire_sm_batcher_create(inspire_sm_server *srv, size_t max_launch_queries,
     b->hbm_budget = hbm_budget_bytes;
     b->n = srv->params.ring_dim;
     b->nb = 2 * (size_t)srv->params.record_bytes;
-    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        sm_err(err, err_cap, "HIP failure creating the batcher stream: %s", hipGetErrorString(e));
-        delete b;
-        return INSPIRE_ERR_DEVICE;
+    auto drop_streams = [b] {
+        for (hipStream_t st : {b->stream, b->copy_in, b->copy_out})
+            if (st) (void)hipStreamDestroy(st);
+    };
+    for (hipStream_t *st : {&b->stream, &b->copy_in, &b->copy_out}) {
+        hipError_t e = hipStreamCreateWithFlags(st, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            sm_err(err, err_cap, "HIP failure creating the batcher streams: %s", hipGetErrorString(e));
+            drop_streams();
+            delete b;
+            return INSPIRE_ERR_DEVICE;
+        }
     }
     for (Slot &s : b->slots) {
         int rc = alloc_slot(b, s, err, err_cap);
         if (rc) {
-            (void)hipStreamDestroy(b->stream);
+            drop_streams();
             delete b;
             return rc;
         }
@@ -247,8 +272,10 @@ void inspire_sm_batcher_destroy(inspire_sm_batcher *b) {
     }
     b->cv_work.notify_all();
     b->worker.join();
-    (void)hipStreamSynchronize(b->stream);
-    (void)hipStreamDestroy(b->stream);
+    for (hipStream_t st : {b->copy_in, b->stream, b->copy_out}) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
     delete b;
 }
 
