@@ -80,29 +80,46 @@ def _timed_queries(run_one, budget_s, threads):
 def cpu_baseline(budget_s=10.0):
     """The CPU oracle (C restatement of the reference respond) on a bounded sample: seeded
     d2048 gamma=256 queries against a prebuilt store, on all of the box's host cores (one
-    query per thread, as a multi-threaded server answers independent requests) and on one."""
+    query per thread, as a multi-threaded server answers independent requests) and on one;
+    with the reference's AVX-512 IFMA multiply-accumulates (its simd-packing-offline feature)
+    where the host has them, and without. value is the all-core rate of the faster form."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
     rng = np.random.default_rng(99)
-    p = oracle.Pack(N, GAMMA)
+    scalar = oracle.Pack(N, GAMMA)
+    simd = oracle.Pack(N, GAMMA)
+    has_simd = simd.set_simd(True)
     cols = rng.integers(0, 1 << 16, (GAMMA, N), dtype=np.uint64)
-    store = p.store_build(cols)
+    store = scalar.store_build(cols)
     y = rng.integers(0, oracle.Q, (3, N), dtype=np.uint64)
     w_seed = bytes(32)
 
-    def one(r):
-        seed = r.integers(0, 256, 32, dtype=np.uint8).tobytes()
-        b = r.integers(0, oracle.Q, N, dtype=np.uint64)
-        p.respond(w_seed, oracle.expand_seed(seed, N), b, cols, y, store)
+    def runner(p):
+        def one(r):
+            seed = r.integers(0, 256, 32, dtype=np.uint8).tobytes()
+            b = r.integers(0, oracle.Q, N, dtype=np.uint64)
+            p.respond(w_seed, oracle.expand_seed(seed, N), b, cols, y, store)
+        return one
 
-    n1, dt1 = _timed_queries(one, budget_s / 2, 1)
     T = _cpu_threads()
-    nT, dtT = _timed_queries(one, budget_s / 2, T)
+    legs = [("scalar", scalar)] + ([("simd", simd)] if has_simd else [])
+    share = budget_s / (2 * len(legs))
+    res = {}
+    for name, p in legs:
+        n1, dt1 = _timed_queries(runner(p), share, 1)
+        nT, dtT = _timed_queries(runner(p), share, T)
+        res[name] = (n1, dt1, nT, dtT)
+    best = "simd" if has_simd else "scalar"
+    n1, dt1, nT, dtT = res[best]
+    sn1, sdt1, snT, sdtT = res["scalar"]
+    form = "with AVX-512 IFMA multiply-accumulates" if has_simd else "scalar (host has no AVX-512 IFMA)"
     return {"value": round(nT / dtT, 3), "unit": "queries/s", "cores": T, "kind": "port",
-            "single_core": round(n1 / dt1, 3),
-            "sample": f"{nT} seeded d2048 gamma=256 queries against a prebuilt store (C restatement), "
-                      f"{T} threads for {dtT:.1f} s; and {n1} on 1 thread for {dt1:.1f} s"}
+            "single_core": round(n1 / dt1, 3), "simd": "avx512ifma" if has_simd else None,
+            "without_simd": {"value": round(snT / sdtT, 3), "single_core": round(sn1 / sdt1, 3)},
+            "sample": f"seeded d2048 gamma=256 queries against a prebuilt store (C restatement), {form}: "
+                      f"{nT} on {T} threads in {dtT:.1f} s and {n1} on 1 thread in {dt1:.1f} s; scalar "
+                      f"{snT} on {T} threads in {sdtT:.1f} s and {sn1} on 1 thread in {sdt1:.1f} s"}
 
 
 def served(g, seconds, rate_qps, seed=7, capacity_queries=0, queries_per_cu=2):

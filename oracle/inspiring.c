@@ -149,7 +149,17 @@ static void const_ntt(const orc_pack *p, uint64_t c, uint64_t *out) {
     orc_ntt_forward(&p->ntt, out);
 }
 
+int orc_pack_set_simd(orc_pack *p, int on) {
+    p->simd = on && p->q == ((1ULL << 60) - (1ULL << 14) + 1) && p->n % 8 == 0 && orc_ifma_available();
+    return p->simd;
+}
+
+/* Poly::mul_acc_ntt_domain (poly.rs:999-1040): the IFMA kernel when switched on */
 static void mul_acc(const orc_pack *p, uint64_t *acc, const uint64_t *a, const uint64_t *b) {
+    if (p->simd) {
+        orc_ifma_mont_mul_acc(acc, a, b, p->n, p->q, p->ntt.q_inv_neg);
+        return;
+    }
     for (size_t j = 0; j < p->n; j++) acc[j] = addmod(acc[j], orc_mont_mul(&p->ntt, a[j], b[j]), p->q);
 }
 
@@ -260,6 +270,13 @@ static void recursion_and_online(const orc_pack *p, const uint64_t *w_mask_ntt, 
         for (size_t k = 0; k < rows; k++) {
             const uint64_t *w = w_mask_ntt + k * n, *y = y_ntt + k * n, *t = dg + k * n;
             uint64_t *r = r_all + i * n;
+            if (p->simd) {
+                /* R_i += T_k w_all[i][k] through mul_acc_ntt_domain (inspiring2.rs:1061); the
+                 * online gather stays a scalar loop (inspiring2.rs:1465-1478) */
+                orc_ifma_mont_mul_acc_gather(r, t, w, tab, n, q, p->ntt.q_inv_neg);
+                for (size_t j = 0; j < n; j++) acc[j] = addmod(acc[j], orc_mont_mul(&p->ntt, y[tab[j]], t[j]), q);
+                continue;
+            }
             for (size_t j = 0; j < n; j++) {
                 /* w_all_ntt[i][k][j] = w_mask_ntt[k][tab_i[j]] (apply_automorphism_ntt) */
                 r[j] = addmod(r[j], orc_mont_mul(&p->ntt, t[j], w[tab[j]]), q);

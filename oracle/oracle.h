@@ -114,9 +114,19 @@ typedef struct {
     uint64_t *neg_mono_ntt;  /* n x n, NTT(-X^j) */
     uint32_t *tables;        /* gamma x n: table of tau_{g^i}, i < gamma */
     orc_ntt ntt;
+    int simd;                /* the reference's AVX-512 IFMA multiply-accumulates (ifma.c) */
 } orc_pack;
 
 int orc_pack_init(orc_pack *p, size_t n, size_t gamma, size_t gadget_len, size_t dropped);
+/* switch the recursion's multiply-accumulates to the IFMA path; returns whether it is on
+ * (it needs the host features, q = 2^60 - 2^14 + 1 and n a multiple of 8) */
+int orc_pack_set_simd(orc_pack *p, int on);
+/* ifma.c */
+int orc_ifma_available(void);
+void orc_ifma_mont_mul_acc(uint64_t *acc, const uint64_t *a, const uint64_t *b, size_t n, uint64_t q,
+                           uint64_t q_inv_neg);
+void orc_ifma_mont_mul_acc_gather(uint64_t *acc, const uint64_t *a, const uint64_t *w, const uint32_t *tab,
+                                  size_t n, uint64_t q, uint64_t q_inv_neg);
 void orc_pack_free(orc_pack *p);
 /* full automorph table for odd t (derive_automorph_tables, inspiring2.rs:330-379) */
 int orc_automorph_table(const orc_pack *p, size_t t, uint32_t *out);

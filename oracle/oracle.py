@@ -69,6 +69,30 @@ _lib.orc_mont_mul.argtypes = [ctypes.POINTER(_Ntt), ctypes.c_uint64, ctypes.c_ui
 _lib.orc_mont_mul.restype = ctypes.c_uint64
 _lib.orc_negacyclic_mul_naive.argtypes = [ctypes.c_size_t, ctypes.c_uint64, _u64p, _u64p, _u64p]
 _lib.orc_chacha_block.argtypes = [_u32p, ctypes.c_uint64, ctypes.c_uint64, _u32p]
+_lib.orc_pack_set_simd.argtypes = [ctypes.POINTER(_Pack), ctypes.c_int]
+_lib.orc_ifma_available.restype = ctypes.c_int
+_lib.orc_ifma_mont_mul_acc.argtypes = [_u64p, _u64p, _u64p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64]
+_lib.orc_ifma_mont_mul_acc_gather.argtypes = [_u64p, _u64p, _u64p, _u32p, ctypes.c_size_t, ctypes.c_uint64,
+                                              ctypes.c_uint64]
+
+
+def ifma_available():
+    """Whether this host has the AVX-512 F/DQ/IFMA features of the reference's SIMD path."""
+    return bool(_lib.orc_ifma_available())
+
+
+def ifma_mont_mul_acc(acc, a, b, q_inv_neg, q=Q, tab=None):
+    """acc + a * b[tab] * 2^-64 mod q through the IFMA kernel (tab None: b itself)."""
+    acc = np.array(acc, dtype=np.uint64)
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    b = np.ascontiguousarray(b, dtype=np.uint64)
+    if tab is None:
+        _lib.orc_ifma_mont_mul_acc(_p64(acc), _p64(a), _p64(b), acc.size, q, q_inv_neg)
+    else:
+        tab = np.ascontiguousarray(tab, dtype=np.uint32)
+        _lib.orc_ifma_mont_mul_acc_gather(_p64(acc), _p64(a), _p64(b), tab.ctypes.data_as(_u32p), acc.size, q,
+                                          q_inv_neg)
+    return acc
 
 
 def _seed(seed):
@@ -122,6 +146,11 @@ class Pack:
 
     def __del__(self):
         _lib.orc_pack_free(ctypes.byref(self._p))
+
+    def set_simd(self, on=True):
+        """Use the reference's AVX-512 IFMA multiply-accumulates in the recursion (the
+        simd-packing-offline feature); returns whether the host runs them."""
+        return bool(_lib.orc_pack_set_simd(ctypes.byref(self._p), 1 if on else 0))
 
     def automorph_table(self, t):
         out = np.zeros(self.n, dtype=np.uint32)
