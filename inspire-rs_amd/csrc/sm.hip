@@ -396,58 +396,75 @@ __global__ __launch_bounds__((1 << LOGN) / 8, 4) void k_sm_respond(SmCtxDev c, S
         rot[k] = (uint32_t)swz(rot_slot(c.g, 8 * t + 2 * k, LOGN)) |
                  (uint32_t)swz(rot_slot(c.g, 8 * t + 2 * k + 1, LOGN)) << 16;
 
+    // The accumulators wait in their permutation buffers (own slots) between uses rather than
+    // in registers across the transforms: B_p is gathered after the first transform and stored
+    // back during the second, past its first barrier; B_q is gathered after the second and stored
+    // after the next step's first barrier, so no store meets another thread's gather.
+    uint32_t bq[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-        BQ[k] = 0;
+        bq[k] = 0;
         BP[k] = 0;
     }
+    own_store(PP, BP, t);
 
     for (int i = c.gamma - 2; i >= 0; i--) {
         const int z = opaque_s(0);
         const Pair32 *twi_q = c.twi_q + z, *twf_q = c.twf_q + z, *twi_p = c.twi_p + z, *twf_p = c.twf_p + z;
-        // Y = tau_g(V), B <- tau_g(B)
+        // Y = tau_g(V); B <- tau_g(B) as each accumulator is next used
         own_store(PV, V, t);
-        own_store(PQ, BQ, t);
-        own_store(PP, BP, t);
         __syncthreads();
+        own_store(PQ, bq, t);
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             const int r = (int)((opaque_v(rot[k >> 1]) >> (16 * (k & 1))) & 0xffffu);
             V[k] = PV[r];
-            BQ[k] = PQ[r];
-            BP[k] = PP[r];
         }
         // tp = NTT_p(lift(iNTT_q(Y)))
         uint32_t v[8], S[8], A[8], WQ[8], KQ[8], WP[8], KP[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) v[k] = V[k];
-        round_trip<LOGN, true>(v, X, c, twi_q, twf_p, t, [&] {
+        // the seat's rows are loaded after the transform, the table row (streamed from HBM)
+        // during it: rows loaded early stay live across the transform, and at the 128-register
+        // budget of four workgroups per CU each one adds spills
+        round_trip<LOGN, true>(v, X, c, twi_q, twf_p, t, [&] {});
+        {
             const uint32_t *sp = seat + opaque_s(0);
             gload8(sp + 2 * N, WP);
             gload8(sp + 3 * N, KP);
-        });
+        }
+        uint32_t bp[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            BP[k] = red(BP[k] + red(mont(v[k], KP[k], p, c.pninv), p), p);
+            const int r = (int)((opaque_v(rot[k >> 1]) >> (16 * (k & 1))) & 0xffffu);
+            bp[k] = red(PP[r] + red(mont(v[k], KP[k], p, c.pninv), p), p);
             v[k] = red(mont(v[k], WP[k], p, c.pninv), p);
         }
         // r = NTT_q(lift(iNTT_p(tp * w_p)))
         round_trip<LOGN, false>(v, X, c, twi_p, twf_q, t, [&] {
+            own_store(PP, bp, t);  // every thread's gather from PP is behind the transform's barriers
             const int z2 = opaque_s(0);
             gload8(job.step1 + (size_t)i * N + 8 * t + z2, S);
+        });
+        {
+            const int z2 = opaque_s(0);
             gload8(apark + z2, A);
             gload8(seat + z2, WQ);
             gload8(seat + N + z2, KQ);
-        });
+        }
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             const uint32_t t1 = mont(A[k], S[k], q, c.qninv);
             const uint32_t t2 = mont(V[k], WQ[k], q, c.qninv);
             const uint32_t t3 = shoup(t2 + 4 * q - v[k], c.pinv_n, q);
-            BQ[k] = red(BQ[k] + mont(V[k], KQ[k], q, c.qninv), 2 * q);
+            const int r = (int)((opaque_v(rot[k >> 1]) >> (16 * (k & 1))) & 0xffffu);
+            bq[k] = red(PQ[r] + mont(V[k], KQ[k], q, c.qninv), 2 * q);
             V[k] = red(t1 + t3, 2 * q);
         }
     }
+#pragma unroll
+    for (int k = 0; k < 8; k++) BQ[k] = bq[k];
+    own_load(PP, BP, t);  // stored by this thread in the last step
     __syncthreads();
 
     // --- body fold (server.rs:398-410): fold[k] = sum_m c_m * nibble(m, k), c_0 = centred(b_0),
