@@ -339,7 +339,7 @@ def special_modulus(g, torch, dev, stream, args):
     }
     if not args.no_served:
         out["served"] = sm_served(g, srv, tables, rng, rate_qps=0.7 * qps, seconds=args.served_seconds,
-                                  capacity_queries=int(max(args.served_seconds, 1.0) * qps))
+                                  capacity_queries=int(max(args.served_seconds, 1.0) * qps), max_launch=L)
     for t in tables:
         t.close()
     for st in seats:
@@ -348,7 +348,7 @@ def special_modulus(g, torch, dev, stream, args):
     return out
 
 
-def sm_served(g, srv, tables, rng, rate_qps, seconds, keys=256, capacity_queries=0):
+def sm_served(g, srv, tables, rng, rate_qps, seconds, keys=256, capacity_queries=0, max_launch=2560):
     """Served special modulus: 256 keys seated by handle in a continuous-batching queue,
     Poisson arrivals of batches of 1-64 queries at uniform tables under uniform keys, from
     host memory (PCIe-inclusive); latency is submit to result. As for the gadget-key served
@@ -356,7 +356,7 @@ def sm_served(g, srv, tables, rng, rate_qps, seconds, keys=256, capacity_queries
     whose drain rate is the batcher's capacity."""
     import threading
 
-    batcher = g.SmBatcher(srv, max_launch=2048)
+    batcher = g.SmBatcher(srv, max_launch=max_launch)
     handles = [batcher.register_key(g.PackingKey(2, rng.bytes(32), rng.integers(0, g.SM_Q, SM_N, dtype=np.uint32),
                                                  rng.integers(0, g.SM_P, SM_N, dtype=np.uint32)))
                for _ in range(keys)]
@@ -431,7 +431,7 @@ def sm_served(g, srv, tables, rng, rate_qps, seconds, keys=256, capacity_queries
             "capacity": capacity,
             "workload": f"{keys} seated keys, Poisson arrivals of 1-64-query batches at uniform tables "
                         f"({len(tables)} resident) and keys at 70% of the batched rate, host inputs "
-                        f"(PCIe-inclusive), continuous batching <= 2048 queries per launch"}
+                        f"(PCIe-inclusive), continuous batching <= {max_launch} queries per launch"}
 
 
 def main():
@@ -453,7 +453,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sm-tables", type=int, default=2048, help="special-modulus step-1 tables (2048 = 2^22 cells)")
     ap.add_argument("--sm-keys", type=int, default=256)
-    ap.add_argument("--sm-batch", type=int, default=2048, help="special-modulus queries per launch")
+    ap.add_argument("--sm-batch", type=int, default=2560,
+                    help="special-modulus queries per launch (2560: two rounds of 5 per CU on 256 CUs)")
     ap.add_argument("--sm-steps", type=int, default=4)
     ap.add_argument("--sm-only", action="store_true", help="run only the special-modulus workload")
     ap.add_argument("--no-sm", action="store_true")

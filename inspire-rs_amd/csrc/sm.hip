@@ -333,11 +333,13 @@ __device__ __forceinline__ int rot_slot(uint32_t t, int j, int logn) {
 // ---------------------------------------------------------------- respond
 
 template <int LOGN>
-__global__ __launch_bounds__((1 << LOGN) / 8, 4) void k_sm_respond(SmCtxDev c, SmBatchDev bt) {
+__global__ __launch_bounds__((1 << LOGN) / 8, 5) void k_sm_respond(SmCtxDev c, SmBatchDev bt) {
     constexpr int N = 1 << LOGN, T = N / 8, NW = Lds<LOGN>::words;
-    __shared__ __align__(16) uint32_t L[4 * NW];
+    // three buffers (27 KiB) so that five workgroups share a CU: the transform buffer X also
+    // serves the rotation of V (PV), the accumulators keep PQ and PP
+    __shared__ __align__(16) uint32_t L[3 * NW];
     __shared__ int rejected;
-    uint32_t *X = L, *PV = L + NW, *PQ = L + 2 * NW, *PP = L + 3 * NW;
+    uint32_t *X = L, *PV = L, *PQ = L + NW, *PP = L + 2 * NW;
     const int qi = blockIdx.x, t = threadIdx.x;
     const SmJob job = bt.jobs[qi];
     const uint32_t q = c.q, p = c.p;
@@ -420,6 +422,7 @@ __global__ __launch_bounds__((1 << LOGN) / 8, 4) void k_sm_respond(SmCtxDev c, S
             const int r = (int)((opaque_v(rot[k >> 1]) >> (16 * (k & 1))) & 0xffffu);
             V[k] = PV[r];
         }
+        __syncthreads();  // every gather from PV is done before the transform reuses it
         // tp = NTT_p(lift(iNTT_q(Y)))
         uint32_t v[8], S[8], A[8], WQ[8], KQ[8], WP[8], KP[8];
 #pragma unroll
@@ -493,7 +496,7 @@ __global__ __launch_bounds__((1 << LOGN) / 8, 4) void k_sm_respond(SmCtxDev c, S
                 s[b] += (int64_t)(cv * nib);
             }
         }
-        int64_t *part = reinterpret_cast<int64_t *>(PV);  // R x 8W = 8T i64 over PV and PQ
+        int64_t *part = reinterpret_cast<int64_t *>(PQ);  // R x 8W = 8T i64 over PQ and PP
 #pragma unroll
         for (int b = 0; b < 8; b++) part[r0 * 8 * W + 8 * w + b] = s[b];
         __syncthreads();
