@@ -310,13 +310,13 @@ def special_modulus(g, torch, dev, stream, args):
     mm_s = L * SM_MM_PER_QUERY / (kms * 1e-3)
     # per query the kernel streams its table (gamma x n words) and reads its cells, seat and body
     bytes_per_query = SM_GAMMA * SM_N * 4 + SM_N * SM_CELL + 4 * SM_N * 4 + SM_N * 4 + SM_N * 4 + 2 * SM_RECORD * 4
-    # HBM bytes per launch measured by rocprofv3 counters for this workload (2048 queries per
+    # HBM bytes per launch measured by rocprofv3 counters for this workload (2560 queries per
     # launch), committed beside the profile; scaled to this run's launch size
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "sm_respond_batched2048_ring2048_cell256_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "sm_respond_batched2560_ring2048_cell256_traffic.json")
     if os.path.exists(tpath):
         tj = json.load(open(tpath))
-        traffic = round(tj["hbm_bytes_per_dispatch"]["inspire::k_sm_respond<11>/2048"] * L / 2048)
+        traffic = round(tj["hbm_bytes_per_dispatch"]["inspire::k_sm_respond<11>/2560"] * L / 2560)
     out = {
         "value": round(qps, 1), "unit": "queries/s",
         "workload": f"special_modulus ring2048_cell256: {len(tables)} step-1 tables ({len(tables) * SM_N} cells of "
