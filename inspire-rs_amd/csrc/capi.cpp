@@ -32,8 +32,9 @@ int build_store(inspire_ctx *ctx, inspire_shard *sh, char *err, size_t err_cap) 
     int rc = sync_table(ctx, err, err_cap);
     if (rc) return rc;
     HIP_TRY(launch_store_build(ctx->dev(), sh->records.as<uint16_t>(), ctx->col_scratch.as<uint16_t>(),
-                               sh->tag_key.as<uint64_t>(), sh->store.as<Pair>(), sh->tags.as<uint64_t>(), 0));
-    HIP_TRY(hipDeviceSynchronize());
+                               sh->tag_key.as<uint64_t>(), sh->store.as<Pair>(), sh->tags.as<uint64_t>(),
+                               ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     sh->has_store = true;
     ShardDev &d = ctx->shards[sh->id];
     d.store = sh->store.as<Pair>();
@@ -171,7 +172,7 @@ int inspire_ctx_create(const inspire_params *params, const uint8_t w_seed[32], u
         inv_p[m] = pair(inv[m]);
     }
 
-    hipError_t e = hipSuccess;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     DevBuf d_seed;
     if (e == hipSuccess) e = ctx->psi.alloc(n * 8);
     if (e == hipSuccess) e = ctx->psi_inv.alloc(n * 8);
@@ -745,8 +746,8 @@ static int respond_host(inspire_ctx *ctx, const inspire_seat *seat, const uint32
         bd.wire_out = ws->wire.as<uint8_t>();
         bd.wire_stride = dstride;
     }
-    HIP_TRY(launch_respond(ctx->dev(), bd, 0));
-    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(launch_respond(ctx->dev(), bd, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     std::vector<int32_t> st(count);
     HIP_TRY(hipMemcpy(st.data(), ws->status.p, count * 4, hipMemcpyDeviceToHost));
     for (size_t k = 0; k < count; k++)

@@ -133,7 +133,14 @@ struct inspire_ctx {
     int queries_per_cu = 1;
     inspire_workspace *default_ws = nullptr;
     std::vector<void *> retired;  // superseded shard tables
+    // the host-buffer entry points and store builds launch here and wait for this stream
+    // alone, not for the whole device (other callers' streams keep running)
+    hipStream_t stream = nullptr;
     ~inspire_ctx() {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
         for (void *p : retired) (void)hipFree(p);
     }
 

@@ -160,7 +160,8 @@ int inspire_sm_server_create(const inspire_sm_params *params, inspire_sm_server 
     inspire_sm_server *s = new (std::nothrow) inspire_sm_server();
     if (!s) { sm_err(err, err_cap, "out of host memory"); return INSPIRE_ERR_DEVICE; }
     s->params = *params;
-    hipError_t e = s->tw.alloc(tw.size() * sizeof(Pair32));
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = s->tw.alloc(tw.size() * sizeof(Pair32));
     if (e == hipSuccess) e = hipMemcpy(s->tw.p, tw.data(), tw.size() * sizeof(Pair32), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = s->cols.alloc(gamma * n);
     if (e != hipSuccess) {
@@ -348,8 +349,8 @@ int inspire_sm_respond_batch(inspire_sm_server *s, size_t count, const inspire_s
     SM_TRY(hipMemcpy(s->bodies.p, bodies, count * n * 4, hipMemcpyHostToDevice));
     SmBatchDev b{(int)count, s->jobs.as<SmJob>(), s->seeds.as<uint8_t>(), s->bodies.as<uint32_t>(),
                  s->mask.as<uint32_t>(), s->body.as<uint32_t>()};
-    SM_TRY(sm_launch_respond(s->dev, b, 0));
-    SM_TRY(hipDeviceSynchronize());
+    SM_TRY(sm_launch_respond(s->dev, b, s->stream));
+    SM_TRY(hipStreamSynchronize(s->stream));
     SM_TRY(hipMemcpy(mask_out, s->mask.p, count * n * 4, hipMemcpyDeviceToHost));
     SM_TRY(hipMemcpy(body_out, s->body.p, count * nb * 4, hipMemcpyDeviceToHost));
     return INSPIRE_OK;

@@ -56,6 +56,14 @@ struct inspire_sm_server {
     inspire_sm::Buf djobs;
     std::vector<inspire::SmJob> djobs_host;
     size_t djobs_cap = 0;
+    // the host-buffer entry points launch here and wait for this stream alone
+    hipStream_t stream = nullptr;
+    ~inspire_sm_server() {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
 };
 
 struct inspire_sm_table {
