@@ -441,12 +441,20 @@ int inspire_server_wait(inspire_server *sv, uint64_t ticket, uint64_t *a_out, ui
                         double *latency_ms, char *err, size_t err_cap) {
     if (!sv) { set_err(err, err_cap, "null server"); return INSPIRE_ERR_INVALID; }
     std::unique_lock<std::mutex> lk(sv->mu);
-    auto it = sv->results.find(ticket);
-    if (it == sv->results.end()) {
+    if (sv->results.find(ticket) == sv->results.end()) {
         set_err(err, err_cap, "unknown ticket %llu", (unsigned long long)ticket);
         return INSPIRE_ERR_INVALID;
     }
-    sv->cv_done.wait(lk, [&] { return it->second.done; });
+    // found again on every wake-up: another waiter on the same ticket may have collected it
+    auto it = sv->results.end();
+    sv->cv_done.wait(lk, [&] {
+        it = sv->results.find(ticket);
+        return it == sv->results.end() || it->second.done;
+    });
+    if (it == sv->results.end()) {
+        set_err(err, err_cap, "ticket %llu was collected by another wait", (unsigned long long)ticket);
+        return INSPIRE_ERR_INVALID;
+    }
     Result res = std::move(it->second);
     sv->results.erase(it);
     lk.unlock();

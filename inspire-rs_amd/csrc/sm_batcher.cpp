@@ -370,12 +370,20 @@ int inspire_sm_batcher_wait(inspire_sm_batcher *b, uint64_t ticket, uint32_t *ma
                             double *latency_ms, char *err, size_t err_cap) {
     if (!b) { sm_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
     std::unique_lock<std::mutex> lk(b->mu);
-    auto it = b->results.find(ticket);
-    if (it == b->results.end()) {
+    if (b->results.find(ticket) == b->results.end()) {
         sm_err(err, err_cap, "unknown ticket %llu", (unsigned long long)ticket);
         return INSPIRE_ERR_INVALID;
     }
-    b->cv_done.wait(lk, [&] { return it->second.done; });
+    // found again on every wake-up: another waiter on the same ticket may have collected it
+    auto it = b->results.end();
+    b->cv_done.wait(lk, [&] {
+        it = b->results.find(ticket);
+        return it == b->results.end() || it->second.done;
+    });
+    if (it == b->results.end()) {
+        sm_err(err, err_cap, "ticket %llu was collected by another wait", (unsigned long long)ticket);
+        return INSPIRE_ERR_INVALID;
+    }
     Result res = std::move(it->second);
     b->results.erase(it);
     lk.unlock();

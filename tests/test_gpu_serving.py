@@ -94,5 +94,27 @@ def test_admission_and_handles(gpu):
         server.submit(hs[0], [0], [bytes(32)], np.zeros((1, N), dtype=np.uint64))
     a, b, _ = server.wait(server.submit(h4, [0], [bytes(32)], np.zeros((1, N), dtype=np.uint64)))
     assert a.shape == (1, N) and b.shape == (1, gamma)
+    # two waits on one ticket: one collects the result, the other is refused (not left waiting)
+    import threading
+    ticket = server.submit(h4, [0, 0], [bytes(32), bytes([1]) * 32], np.zeros((2, N), dtype=np.uint64))
+    outcomes = []
+
+    def collect():
+        try:
+            server.wait(ticket)
+            outcomes.append("ok")
+        except gpu.PirError as e:
+            outcomes.append(str(e))
+
+    ths = [threading.Thread(target=collect) for _ in range(2)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join(timeout=60)
+    assert sorted(outcomes, key=lambda o: o != "ok")[0] == "ok" and len(outcomes) == 2
+    refused = [o for o in outcomes if o != "ok"][0]
+    assert "collected by another wait" in refused or "unknown ticket" in refused  # which depends on timing
+    with pytest.raises(gpu.PirError, match="unknown ticket"):
+        server.wait(ticket)
     server.close()
     ctx.close()
