@@ -32,6 +32,8 @@ extern "C" {
 
 #define INSPIRE_Q 1152921504606830593ULL      /* 2^60 - 2^14 + 1, src/math/mod_q.rs:7 */
 #define INSPIRE_Q36 68718428161ULL            /* MOD_SWITCH_TARGET_36BIT, src/pir/mod_switch.rs:47 */
+/* the most queries one call, workspace or launch takes (2^22): beyond it a call is refused */
+#define INSPIRE_MAX_BATCH 4194304u
 
 typedef struct inspire_ctx inspire_ctx;
 typedef struct inspire_shard inspire_shard;

@@ -185,6 +185,10 @@ int build_store(inspire_ctx *ctx, inspire_shard *sh, char *err, size_t err_cap);
 inline int ws_reserve(inspire_workspace *ws, size_t count, char *err, size_t err_cap) {
     inspire_ctx *ctx = ws->ctx;
     if (count <= ws->max_count && ws->scan_part.p) return INSPIRE_OK;
+    if (count > INSPIRE_MAX_BATCH) {
+        set_err(err, err_cap, "a workspace holds at most %u queries, asked for %zu", INSPIRE_MAX_BATCH, count);
+        return INSPIRE_ERR_INVALID;
+    }
     size_t c = count < 1 ? 1 : count;
     HIP_TRY(ws->scan_part.alloc(c * ctx->ks() * ctx->gamma * 4 * sizeof(uint64_t)));
     HIP_TRY(ws->status.alloc(c * sizeof(int32_t)));

@@ -273,8 +273,8 @@ extern "C" {
 
 int inspire_server_create(inspire_ctx *ctx, size_t max_launch_queries, size_t hbm_budget_bytes, inspire_server **out,
                           char *err, size_t err_cap) {
-    if (!ctx || !out || max_launch_queries == 0) {
-        set_err(err, err_cap, "server needs a context and a launch size of at least one query");
+    if (!ctx || !out || max_launch_queries == 0 || max_launch_queries > INSPIRE_MAX_BATCH) {
+        set_err(err, err_cap, "server needs a context and a launch size of 1 to 4194304 queries");
         return INSPIRE_ERR_INVALID;
     }
     inspire_server *sv = new (std::nothrow) inspire_server();

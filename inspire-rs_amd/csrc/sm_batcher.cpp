@@ -227,8 +227,8 @@ extern "C" {
 
 int inspire_sm_batcher_create(inspire_sm_server *srv, size_t max_launch_queries, size_t hbm_budget_bytes,
                               inspire_sm_batcher **out, char *err, size_t err_cap) {
-    if (!srv || !out || max_launch_queries == 0) {
-        sm_err(err, err_cap, "a batcher needs a server and a launch size of at least one query");
+    if (!srv || !out || max_launch_queries == 0 || max_launch_queries > INSPIRE_MAX_BATCH) {
+        sm_err(err, err_cap, "a batcher needs a server and a launch size of 1 to 4194304 queries");
         return INSPIRE_ERR_INVALID;
     }
     inspire_sm_batcher *b = new (std::nothrow) inspire_sm_batcher();

@@ -329,6 +329,10 @@ int inspire_sm_respond_batch(inspire_sm_server *s, size_t count, const inspire_s
         return INSPIRE_ERR_INVALID;
     }
     if (count == 0) return INSPIRE_OK;
+    if (count > INSPIRE_MAX_BATCH) {
+        sm_err(err, err_cap, "a batch holds at most %u queries, got %zu", INSPIRE_MAX_BATCH, count);
+        return INSPIRE_ERR_INVALID;
+    }
     const size_t n = s->params.ring_dim, nb = 2 * (size_t)s->params.record_bytes;
     std::vector<SmJob> jobs;
     int rc = check_handles(s, count, tables, seats, jobs, err, err_cap);
@@ -371,6 +375,10 @@ int inspire_sm_respond_batch_device(inspire_sm_server *s, size_t count, const in
         return INSPIRE_ERR_INVALID;
     }
     if (count == 0) return INSPIRE_OK;
+    if (count > INSPIRE_MAX_BATCH) {
+        sm_err(err, err_cap, "a batch holds at most %u queries, got %zu", INSPIRE_MAX_BATCH, count);
+        return INSPIRE_ERR_INVALID;
+    }
     std::vector<SmJob> jobs;
     int rc = check_handles(s, count, tables, seats, jobs, err, err_cap);
     if (rc) return rc;

@@ -140,3 +140,8 @@ def test_refusals(gpu):
         with pytest.raises(gpu.PirError, match="not canonical"):
             ctx.respond_seeded_batch(seat, [0], [bytes(32)], bad)
         seat.close()
+        # sizes past INSPIRE_MAX_BATCH are refused before anything is allocated
+        with pytest.raises(gpu.PirError, match="at most 4194304 queries"):
+            gpu.Workspace(ctx, (1 << 22) + 1)
+        with pytest.raises(gpu.PirError, match="1 to 4194304 queries"):
+            gpu.Server(ctx, max_launch=(1 << 22) + 1)

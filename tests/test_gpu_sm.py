@@ -156,6 +156,8 @@ def test_refusals(gpu):
         foreign = other.table(b"")
         with pytest.raises(gpu.PirError, match="another parameter set"):
             server.respond(foreign, seat, _random_query(gpu, rng, 2048))
+        with pytest.raises(gpu.PirError, match="1 to 4194304 queries"):
+            gpu.SmBatcher(server, max_launch=(1 << 22) + 1)
         for x in (table, seat, foreign):
             x.close()
     bad_params = gpu.ring2048_cell256()
