@@ -669,10 +669,12 @@ static int respond_host(inspire_ctx *ctx, const inspire_seat *seat, const uint32
         set_err(err, err_cap, "null argument");
         return INSPIRE_ERR_INVALID;
     }
-    if (count == 0) return INSPIRE_OK;
     std::lock_guard<std::mutex> lk(ctx->mu);
+    // the seat is checked even for an empty batch, as respond_seeded_batch resolves its key
+    // before mapping over the queries (respond.rs:1189-1202)
     int rc = check_batch(ctx, seat, shard_ids, count, err, err_cap);
     if (rc) return rc;
+    if (count == 0) return INSPIRE_OK;
     const size_t n = ctx->n, g = ctx->gamma;
     for (size_t k = 0; k < count; k++)
         for (size_t j = 0; j < n; j++)

@@ -145,3 +145,31 @@ def test_refusals(gpu):
             gpu.Workspace(ctx, (1 << 22) + 1)
         with pytest.raises(gpu.PirError, match="1 to 4194304 queries"):
             gpu.Server(ctx, max_launch=(1 << 22) + 1)
+
+
+def test_empty_batch(gpu):
+    """respond_seeded_batch over no queries answers an empty list, after resolving its key
+    (respond.rs:1189-1202): a seat of another context is refused even then."""
+    import torch
+
+    n, gamma = 2048, 16
+    with gpu.Context(gpu.params_d2048(), bytes(32), gamma) as ctx, \
+            gpu.Context(gpu.params_d2048(), bytes(32), gamma) as other:
+        ctx.upload_columns(0, np.zeros((gamma, n), dtype=np.uint64))
+        seat = ctx.seat(np.zeros((3, n), dtype=np.uint64))
+        foreign = other.seat(np.zeros((3, n), dtype=np.uint64))
+        a, b = ctx.respond_seeded_batch(seat, [], [], np.zeros((0, n), dtype=np.uint64))
+        assert a.shape == (0, n) and b.shape == (0, gamma)
+        with pytest.raises(gpu.PirError, match="another context"):
+            ctx.respond_seeded_batch(foreign, [], [], np.zeros((0, n), dtype=np.uint64))
+        ws = gpu.Workspace(ctx, 4)
+        dummy = torch.zeros(8, dtype=torch.int64, device="cuda:0")
+        ctx.respond_batch_device(seat, ws, dummy.data_ptr(), dummy.data_ptr(), dummy.data_ptr(), 0,
+                                 dummy.data_ptr(), dummy.data_ptr())
+        torch.cuda.synchronize()
+        with pytest.raises(gpu.PirError, match="another context"):
+            ctx.respond_batch_device(foreign, ws, dummy.data_ptr(), dummy.data_ptr(), dummy.data_ptr(), 0,
+                                     dummy.data_ptr(), dummy.data_ptr())
+        ws.close()
+        foreign.close()
+        seat.close()
