@@ -274,6 +274,22 @@ int inspire_sm_respond_batch_device(inspire_sm_server *server, size_t count,
                                     const uint8_t *d_seeds, const uint32_t *d_bodies, uint32_t *d_mask_out,
                                     uint32_t *d_body_out, void *stream, float *kernel_ms, char *err,
                                     size_t err_cap);
+/* Length of one Response::to_bytes (wire.rs:125-131): the mask then the body, each packed
+ * LSB-first at the bit width of q - 1 (9,208 B at ring2048_cell256). */
+size_t inspire_sm_response_wire_bytes(const inspire_sm_server *server);
+/* inspire_sm_respond_batch returning each query's Response::to_bytes, packed by the kernel:
+ * query k's bytes at wire_out + k * wire_stride (wire_stride >= the response length). */
+int inspire_sm_respond_batch_wire(inspire_sm_server *server, size_t count, const inspire_sm_table *const *tables,
+                                  const inspire_sm_seat *const *seats, const uint8_t *seeds, const uint32_t *bodies,
+                                  uint8_t *wire_out, size_t wire_stride, char *err, size_t err_cap);
+/* The same on device buffers, launched on stream. d_scratch holds count * ring_dim words that
+ * the chain uses while it runs (the mask output's role in inspire_sm_respond_batch_device);
+ * wire_stride is a multiple of 4 at least the response length rounded up to 4. */
+int inspire_sm_respond_batch_device_wire(inspire_sm_server *server, size_t count,
+                                         const inspire_sm_table *const *tables, const inspire_sm_seat *const *seats,
+                                         const uint8_t *d_seeds, const uint32_t *d_bodies, uint32_t *d_scratch,
+                                         uint8_t *d_wire_out, size_t wire_stride, void *stream, char *err,
+                                         size_t err_cap);
 
 /* Continuous-batching queue over one special-modulus server: requests (a batch of
  * queries under one seated key, each query naming its own table) from any thread are

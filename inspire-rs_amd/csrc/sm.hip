@@ -518,9 +518,25 @@ __global__ __launch_bounds__((1 << LOGN) / 8, 5) void k_sm_respond(SmCtxDev c, S
     inverse_to_lds<LOGN, false>(V, X, c.twi_q, q, t);
     inverse_to_lds<LOGN, false>(BQ, PQ, c.twi_q, q, t);
     inverse_to_lds<LOGN, true>(BP, PP, c.twi_p, p, t);
-    uint32_t *mask = bt.mask_out + (size_t)qi * N;
-    for (int j = t; j < N; j += T) mask[j] = red(X[swz(j)], q);
-    uint32_t *bout = bt.body_out + (size_t)qi * c.nb;
+    if (!bt.wire_out) {
+        uint32_t *mask = bt.mask_out + (size_t)qi * N;
+        for (int j = t; j < N; j += T) mask[j] = red(X[swz(j)], q);
+        uint32_t *bout = bt.body_out + (size_t)qi * c.nb;
+#pragma unroll
+        for (int j = 0; j < FK; j++) {
+            const int k = t + j * T;
+            if (k < c.nb) {
+                const uint32_t xq = red(PQ[swz(k)], q);
+                const uint32_t xp = PP[swz(k)];
+                const uint32_t d = xq + 3 * q - lift_pq(xp, c);  // (0, 4q)
+                bout[k] = red(red(shoup(d, c.pinv, q), q) + F[j], q);
+            }
+        }
+        return;
+    }
+    // the response bytes: the final mask in place in X, the body in PQ (each thread its own
+    // coefficients), then one 4-byte word per lane and pass
+    for (int j = t; j < N; j += T) X[swz(j)] = red(X[swz(j)], q);
 #pragma unroll
     for (int j = 0; j < FK; j++) {
         const int k = t + j * T;
@@ -528,8 +544,33 @@ __global__ __launch_bounds__((1 << LOGN) / 8, 5) void k_sm_respond(SmCtxDev c, S
             const uint32_t xq = red(PQ[swz(k)], q);
             const uint32_t xp = PP[swz(k)];
             const uint32_t d = xq + 3 * q - lift_pq(xp, c);  // (0, 4q)
-            bout[k] = red(red(shoup(d, c.pinv, q), q) + F[j], q);
+            PQ[swz(k)] = red(red(shoup(d, c.pinv, q), q) + F[j], q);
         }
+    }
+    __syncthreads();
+    const SmWire WL = sm_wire(N, c.nb, q);
+    // byte p of `count` coefficients packed LSB-first at WL.w bits (w >= 8: a byte spans at
+    // most two coefficients); zero past the last one
+    auto tight = [&](const uint32_t *V, uint32_t count, uint32_t p) -> uint32_t {
+        const uint32_t bit = 8 * p, i = bit / WL.w, sh = bit - i * WL.w;
+        if (i >= count) return 0u;
+        uint64_t v = V[swz((int)i)] >> sh;
+        if (WL.w - sh < 8 && i + 1 < count) v |= (uint64_t)V[swz((int)i + 1)] << (WL.w - sh);
+        return (uint32_t)(v & 0xff);
+    };
+    uint32_t *out = reinterpret_cast<uint32_t *>(bt.wire_out + (size_t)qi * bt.wire_stride);
+    const uint32_t words = (WL.total + 3) / 4;
+    for (uint32_t wi = t; wi < words; wi += T) {
+        uint32_t word = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t o = 4 * wi + k;
+            const uint32_t byte = o < WL.mask_bytes ? tight(X, N, o)
+                                  : o < WL.total     ? tight(PQ, (uint32_t)c.nb, o - WL.mask_bytes)
+                                                    : 0u;
+            word |= byte << (8 * k);
+        }
+        out[wi] = word;
     }
 }
 

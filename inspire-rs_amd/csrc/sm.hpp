@@ -46,9 +46,29 @@ struct SmBatchDev {
     const SmJob *jobs;
     const uint8_t *seeds;    // count x 32
     const uint32_t *bodies;  // count x n
-    uint32_t *mask_out;      // count x n
-    uint32_t *body_out;      // count x nb
+    uint32_t *mask_out;      // count x n (also where each query parks NTT(a) during the chain)
+    uint32_t *body_out;      // count x nb, unused when wire_out is set
+    // when non-null, each query's Response::to_bytes at wire_out + qi * wire_stride (stride a
+    // multiple of 4 holding the length rounded up to 4); mask_out then is scratch only
+    uint8_t *wire_out = nullptr;
+    size_t wire_stride = 0;
 };
+
+// Response::to_bytes (wire.rs:125-131): mask then body, each packed LSB-first at the bit
+// width of q - 1 (pack_coefficients_tight, poly.rs:166-225)
+struct SmWire {
+    uint32_t w, mask_bytes, body_bytes, total;
+};
+__host__ __device__ inline SmWire sm_wire(uint32_t n, uint32_t nb, uint32_t q) {
+    SmWire L;
+    uint32_t w = 0;
+    while (w < 32 && ((q - 1) >> w) != 0) w++;
+    L.w = w;
+    L.mask_bytes = (n * w + 7) / 8;
+    L.body_bytes = (nb * w + 7) / 8;
+    L.total = L.mask_bytes + L.body_bytes;
+    return L;
+}
 
 // Step-1 table of one shard from its cells (device), using cols (gamma x n bytes) as scratch.
 hipError_t sm_launch_table(const SmCtxDev &c, const uint8_t *cells, int8_t *cols, uint32_t *step1, hipStream_t s);

@@ -366,6 +366,112 @@ int inspire_sm_respond(inspire_sm_server *s, const inspire_sm_table *table, cons
     return inspire_sm_respond_batch(s, 1, &table, &seat, seed, body, mask_out, body_out, err, err_cap);
 }
 
+// The device entry points' job list, uploaded only when the handles change, so a repeated
+// batch launches with nothing but the kernel on the stream. A launch of this server on any
+// stream may still read the old list, so a change waits for the device first. Caller holds s->mu.
+static int upload_device_jobs(inspire_sm_server *s, const std::vector<SmJob> &jobs, char *err, size_t err_cap) {
+    if (jobs.size() == s->djobs_host.size() &&
+        std::memcmp(jobs.data(), s->djobs_host.data(), jobs.size() * sizeof(SmJob)) == 0)
+        return INSPIRE_OK;
+    SM_TRY(hipDeviceSynchronize());
+    if (jobs.size() > s->djobs_cap) {
+        SM_TRY(s->djobs.alloc(jobs.size() * sizeof(SmJob)));
+        s->djobs_cap = jobs.size();
+    }
+    SM_TRY(hipMemcpy(s->djobs.p, jobs.data(), jobs.size() * sizeof(SmJob), hipMemcpyHostToDevice));
+    s->djobs_host = jobs;
+    return INSPIRE_OK;
+}
+
+size_t inspire_sm_response_wire_bytes(const inspire_sm_server *s) {
+    if (!s) return 0;
+    return sm_wire((uint32_t)s->params.ring_dim, 2 * (uint32_t)s->params.record_bytes, (uint32_t)s->params.q).total;
+}
+
+int inspire_sm_respond_batch_wire(inspire_sm_server *s, size_t count, const inspire_sm_table *const *tables,
+                                  const inspire_sm_seat *const *seats, const uint8_t *seeds, const uint32_t *bodies,
+                                  uint8_t *wire_out, size_t wire_stride, char *err, size_t err_cap) {
+    if (!s || (count && (!tables || !seats || !seeds || !bodies || !wire_out))) {
+        sm_err(err, err_cap, "null argument");
+        return INSPIRE_ERR_INVALID;
+    }
+    if (count == 0) return INSPIRE_OK;
+    if (count > INSPIRE_MAX_BATCH) {
+        sm_err(err, err_cap, "a batch holds at most %u queries, got %zu", INSPIRE_MAX_BATCH, count);
+        return INSPIRE_ERR_INVALID;
+    }
+    const size_t n = s->params.ring_dim;
+    const SmWire wl = sm_wire((uint32_t)n, 2 * (uint32_t)s->params.record_bytes, (uint32_t)s->params.q);
+    if (wire_stride < wl.total) {
+        sm_err(err, err_cap, "response stride %zu is below the %u-byte response", wire_stride, wl.total);
+        return INSPIRE_ERR_INVALID;
+    }
+    std::vector<SmJob> jobs;
+    int rc = check_handles(s, count, tables, seats, jobs, err, err_cap);
+    if (rc) return rc;
+    for (size_t k = 0; k < count * n; k++)
+        if (bodies[k] >= s->params.q) { sm_err(err, err_cap, "query does not fit the set"); return INSPIRE_ERR_INVALID; }
+    const size_t dstride = (wl.total + 3) & ~(size_t)3;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (count > s->cap) {
+        SM_TRY(s->jobs.alloc(count * sizeof(SmJob)));
+        SM_TRY(s->seeds.alloc(count * 32));
+        SM_TRY(s->bodies.alloc(count * n * 4));
+        SM_TRY(s->mask.alloc(count * n * 4));
+        SM_TRY(s->body.alloc(count * 2 * (size_t)s->params.record_bytes * 4));
+        s->cap = count;
+    }
+    if (s->wire_cap < count * dstride) {
+        SM_TRY(s->wire.alloc(count * dstride));
+        s->wire_cap = count * dstride;
+    }
+    SM_TRY(hipMemcpy(s->jobs.p, jobs.data(), count * sizeof(SmJob), hipMemcpyHostToDevice));
+    SM_TRY(hipMemcpy(s->seeds.p, seeds, count * 32, hipMemcpyHostToDevice));
+    SM_TRY(hipMemcpy(s->bodies.p, bodies, count * n * 4, hipMemcpyHostToDevice));
+    SmBatchDev b{(int)count, s->jobs.as<SmJob>(), s->seeds.as<uint8_t>(), s->bodies.as<uint32_t>(),
+                 s->mask.as<uint32_t>(), nullptr};
+    b.wire_out = s->wire.as<uint8_t>();
+    b.wire_stride = dstride;
+    SM_TRY(sm_launch_respond(s->dev, b, s->stream));
+    SM_TRY(hipStreamSynchronize(s->stream));
+    SM_TRY(hipMemcpy2D(wire_out, wire_stride, s->wire.p, dstride, wl.total, count, hipMemcpyDeviceToHost));
+    return INSPIRE_OK;
+}
+
+int inspire_sm_respond_batch_device_wire(inspire_sm_server *s, size_t count, const inspire_sm_table *const *tables,
+                                         const inspire_sm_seat *const *seats, const uint8_t *d_seeds,
+                                         const uint32_t *d_bodies, uint32_t *d_scratch, uint8_t *d_wire_out,
+                                         size_t wire_stride, void *stream, char *err, size_t err_cap) {
+    if (!s || (count && (!tables || !seats || !d_seeds || !d_bodies || !d_scratch || !d_wire_out))) {
+        sm_err(err, err_cap, "null argument");
+        return INSPIRE_ERR_INVALID;
+    }
+    if (count == 0) return INSPIRE_OK;
+    if (count > INSPIRE_MAX_BATCH) {
+        sm_err(err, err_cap, "a batch holds at most %u queries, got %zu", INSPIRE_MAX_BATCH, count);
+        return INSPIRE_ERR_INVALID;
+    }
+    const size_t n = s->params.ring_dim;
+    const SmWire wl = sm_wire((uint32_t)n, 2 * (uint32_t)s->params.record_bytes, (uint32_t)s->params.q);
+    if (wire_stride < ((wl.total + 3) & ~3u) || wire_stride % 4 != 0) {
+        sm_err(err, err_cap, "response stride %zu must be a multiple of 4 holding %u bytes", wire_stride,
+               (wl.total + 3) & ~3u);
+        return INSPIRE_ERR_INVALID;
+    }
+    std::vector<SmJob> jobs;
+    int rc = check_handles(s, count, tables, seats, jobs, err, err_cap);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lk(s->mu);
+    rc = upload_device_jobs(s, jobs, err, err_cap);
+    if (rc) return rc;
+    SmBatchDev b{(int)count, s->djobs.as<SmJob>(), d_seeds, d_bodies, d_scratch, nullptr};
+    b.wire_out = d_wire_out;
+    b.wire_stride = wire_stride;
+    SM_TRY(sm_launch_respond(s->dev, b, st));
+    return INSPIRE_OK;
+}
+
 int inspire_sm_respond_batch_device(inspire_sm_server *s, size_t count, const inspire_sm_table *const *tables,
                                     const inspire_sm_seat *const *seats, const uint8_t *d_seeds,
                                     const uint32_t *d_bodies, uint32_t *d_mask_out, uint32_t *d_body_out,
@@ -384,20 +490,8 @@ int inspire_sm_respond_batch_device(inspire_sm_server *s, size_t count, const in
     if (rc) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
     std::lock_guard<std::mutex> lk(s->mu);
-    // the job list is uploaded only when the handles change, so a repeated batch
-    // launches with nothing but the kernel on the stream
-    bool same = jobs.size() == s->djobs_host.size() &&
-                std::memcmp(jobs.data(), s->djobs_host.data(), jobs.size() * sizeof(SmJob)) == 0;
-    if (!same) {
-        if (count > s->djobs_cap) {
-            SM_TRY(hipStreamSynchronize(st));
-            SM_TRY(s->djobs.alloc(count * sizeof(SmJob)));
-            s->djobs_cap = count;
-        }
-        SM_TRY(hipStreamSynchronize(st));
-        SM_TRY(hipMemcpy(s->djobs.p, jobs.data(), count * sizeof(SmJob), hipMemcpyHostToDevice));
-        s->djobs_host = jobs;
-    }
+    rc = upload_device_jobs(s, jobs, err, err_cap);
+    if (rc) return rc;
     SmBatchDev b{(int)count, s->djobs.as<SmJob>(), d_seeds, d_bodies, d_mask_out, d_body_out};
     if (!kernel_ms) {
         SM_TRY(sm_launch_respond(s->dev, b, st));

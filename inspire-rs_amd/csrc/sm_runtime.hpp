@@ -56,6 +56,9 @@ struct inspire_sm_server {
     inspire_sm::Buf djobs;
     std::vector<inspire::SmJob> djobs_host;
     size_t djobs_cap = 0;
+    // staged response bytes of the host wire entry point
+    inspire_sm::Buf wire;
+    size_t wire_cap = 0;
     // the host-buffer entry points launch here and wait for this stream alone
     hipStream_t stream = nullptr;
     ~inspire_sm_server() {

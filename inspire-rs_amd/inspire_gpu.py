@@ -119,6 +119,10 @@ def lib():
         L.inspire_sm_respond.argtypes = [vp, vp, vp, vp, vp, vp, vp, cp, sz]
         L.inspire_sm_respond_batch.argtypes = [vp, sz, vp, vp, vp, vp, vp, vp, cp, sz]
         L.inspire_sm_respond_batch_device.argtypes = [vp, sz, vp, vp, vp, vp, vp, vp, vp, vp, cp, sz]
+        L.inspire_sm_response_wire_bytes.argtypes = [vp]
+        L.inspire_sm_response_wire_bytes.restype = sz
+        L.inspire_sm_respond_batch_wire.argtypes = [vp, sz, vp, vp, vp, vp, vp, sz, cp, sz]
+        L.inspire_sm_respond_batch_device_wire.argtypes = [vp, sz, vp, vp, vp, vp, vp, vp, sz, vp, cp, sz]
         L.inspire_sm_batcher_create.argtypes = [vp, sz, sz, pp, cp, sz]
         L.inspire_sm_batcher_destroy.argtypes = [vp]
         L.inspire_sm_batcher_register_key.argtypes = [vp, vp, vp, vp, ctypes.POINTER(u64), cp, sz]
@@ -694,6 +698,37 @@ class SmServer:
         _call(lib().inspire_sm_respond_batch, self._h, count, th, sh, _ptr(seeds), _ptr(bodies), _ptr(masks),
               _ptr(outs))
         return masks, outs
+
+    def response_bytes(self):
+        """Length of one Response::to_bytes."""
+        return lib().inspire_sm_response_wire_bytes(self._h)
+
+    def respond_batch_wire(self, tables, seats, queries):
+        """respond_batch returning each query's Response::to_bytes, packed on the device."""
+        count = len(queries)
+        if count == 0:
+            return []
+        for qy in queries:
+            if qy.body.size != self.params.ring_dim:
+                raise PirError("query does not fit the set")
+        th = (ctypes.c_void_p * count)(*[t._h.value for t in tables])
+        sh = (ctypes.c_void_p * count)(*[s._h.value for s in seats])
+        seeds = np.frombuffer(b"".join(qy.seed for qy in queries), dtype=np.uint8).copy()
+        bodies = np.ascontiguousarray(np.stack([qy.body for qy in queries]), dtype=np.uint32)
+        size = self.response_bytes()
+        out = np.zeros((count, size), np.uint8)
+        _call(lib().inspire_sm_respond_batch_wire, self._h, count, th, sh, _ptr(seeds), _ptr(bodies), _ptr(out),
+              size)
+        return [out[k].tobytes() for k in range(count)]
+
+    def respond_batch_device_wire(self, tables, seats, d_seeds, d_bodies, d_scratch, d_wire, wire_stride, stream=0):
+        """Device-resident inputs, scratch (count x ring_dim words) and response bytes."""
+        count = len(tables)
+        th = (ctypes.c_void_p * count)(*[t._h.value for t in tables])
+        sh = (ctypes.c_void_p * count)(*[s._h.value for s in seats])
+        _call(lib().inspire_sm_respond_batch_device_wire, self._h, count, th, sh, ctypes.c_void_p(d_seeds),
+              ctypes.c_void_p(d_bodies), ctypes.c_void_p(d_scratch), ctypes.c_void_p(d_wire), wire_stride,
+              ctypes.c_void_p(stream))
 
     def respond_batch_device(self, tables, seats, d_seeds, d_bodies, d_mask, d_body, stream=0, timed=False):
         """Device-resident inputs and outputs (addresses); returns the kernel ms when timed."""

@@ -47,6 +47,10 @@ def test_served_wire_bytes_are_pinned(gpu):
         assert (len(key_bytes), len(query_bytes), len(response_bytes)) == (15_394, 7_456, 9_208)
         assert [oracle.fnv1a(key_bytes), oracle.fnv1a(query_bytes), oracle.fnv1a(response_bytes)] == PINNED
         assert client.decode(response.mask, response.body) == bytes(cells[1234 * 256:1234 * 256 + 246])
+        # the same response packed by the kernel (Response::to_bytes on the device)
+        assert server.response_bytes() == 9_208
+        wire = server.respond_batch_wire([table], [seat], [gpu.Query.from_bytes(params, query_bytes)])[0]
+        assert oracle.fnv1a(wire) == PINNED[2]
         seat.close()
         table.close()
 
@@ -164,3 +168,48 @@ def test_refusals(gpu):
     bad_params.key_mask_version = 1
     with pytest.raises(gpu.PirError, match="per-key-seed version"):
         gpu.SmServer(bad_params)
+
+
+@pytest.mark.parametrize("d,gamma", [(2048, 512), (256, 32), (64, 8)])
+def test_device_wire_bytes_equal_the_host_codec(gpu, d, gamma):
+    """Response::to_bytes packed by the kernel equals the host codec over the word arrays of
+    the same batch, through host and device buffers (uneven strides, mixed tables and keys)."""
+    import torch
+
+    params = gpu.ring2048_cell256() if d == 2048 else _small(gpu, d, gamma)
+    rng = np.random.default_rng(d + gamma)
+    with gpu.SmServer(params) as server:
+        def cells(rows):
+            c = rng.integers(0, 256, (rows, params.cell_bytes), dtype=np.uint8)
+            c[:, params.record_bytes:] = 0  # a cell holds nothing past its record
+            return c.tobytes()
+
+        tables = [server.table(cells(int(rng.integers(1, d + 1)))) for _ in range(3)]
+        seats = [server.seat(_random_key(gpu, rng, d)) for _ in range(2)]
+        count = 7
+        tl = [tables[int(i)] for i in rng.integers(0, 3, count)]
+        sl = [seats[int(i)] for i in rng.integers(0, 2, count)]
+        queries = [_random_query(gpu, rng, d) for _ in range(count)]
+        masks, bodies = server.respond_batch(tl, sl, queries)
+        want = [gpu.Response(masks[k], bodies[k]).to_bytes(params) for k in range(count)]
+        assert [len(w) for w in want] == [server.response_bytes()] * count
+        assert server.respond_batch_wire(tl, sl, queries) == want
+        size = server.response_bytes()
+        stride = (size + 3) // 4 * 4 + 8
+        dev = torch.device("cuda:0")
+        d_seeds = torch.from_numpy(np.frombuffer(b"".join(q.seed for q in queries), np.uint8).copy()).to(dev)
+        d_bodies = torch.from_numpy(np.stack([q.body for q in queries]).astype(np.uint32).view(np.int32)).to(dev)
+        d_scratch = torch.zeros((count, d), dtype=torch.int32, device=dev)
+        d_wire = torch.full((count, stride), 0xAB, dtype=torch.uint8, device=dev)
+        server.respond_batch_device_wire(tl, sl, d_seeds.data_ptr(), d_bodies.data_ptr(), d_scratch.data_ptr(),
+                                         d_wire.data_ptr(), stride)
+        torch.cuda.synchronize()
+        got = d_wire.cpu().numpy()
+        for k in range(count):
+            assert got[k, :size].tobytes() == want[k], f"query {k}"
+            assert (got[k, (size + 3) // 4 * 4:] == 0xAB).all()  # nothing written past the padded length
+        with pytest.raises(gpu.PirError, match="multiple of 4"):
+            server.respond_batch_device_wire(tl, sl, d_seeds.data_ptr(), d_bodies.data_ptr(), d_scratch.data_ptr(),
+                                             d_wire.data_ptr(), size + 1)
+        for x in tables + seats:
+            x.close()
