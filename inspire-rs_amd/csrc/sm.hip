@@ -25,7 +25,8 @@
 // companions, a*w - floor(a*w'/2^32)*m; products by the per-query rows (the table,
 // the seat) are Montgomery products a*b*2^-32 with the rows stored as b = x*2^32, so
 // they need no companion words in HBM or registers. Both land in [0, 2m).
-// Residues mod q stay lazy in [0, 4q) (q < 2^30); residues mod p (p < 2^31) stay canonical.
+// Residues mod q stay lazy in [0, 4q) (q < 2^30); residues mod p (p < 2^31) stay canonical,
+// except the forward transforms' outputs, in [0, 2p).
 //
 // Layout: 8 slots per thread. Radix-8 passes over bits 0-2 (the thread's own slots
 // 8t..8t+7), 3-5, 6-8 ..., the top pass taking the remaining 1-3 bits; each round trip
@@ -65,17 +66,18 @@ __device__ __forceinline__ void ct_q(uint32_t &x, uint32_t &y, Pair32 w, uint32_
     x = u + t;
     y = u - t + 2 * q;
 }
-// mod p, canonical
+// mod p (4p overflows a word): GS canonical -> canonical; CT (Harvey) on [0, 2p) -> [0, 2p),
+// since every consumer of a forward transform mod p (a Montgomery or Shoup product) takes
+// any word
 __device__ __forceinline__ void gs_p(uint32_t &x, uint32_t &y, Pair32 w, uint32_t p) {
     const uint32_t s = x + y, d = x - y + p;
     x = red(s, p);
     y = red(shoup(d, w, p), p);
 }
 __device__ __forceinline__ void ct_p(uint32_t &x, uint32_t &y, Pair32 w, uint32_t p) {
-    const uint32_t t = red(shoup(y, w, p), p);
-    const uint32_t s = x + t, d = x - t + p;
-    x = red(s, p);
-    y = red(d, p);
+    const uint32_t u = red(x, p), t = red(shoup(y, w, p), p);
+    x = u + t;
+    y = u - t + p;
 }
 
 template <int LOGN>
@@ -255,7 +257,7 @@ __device__ __forceinline__ uint32_t lift_pq(uint32_t r, const SmCtxDev &c) {
 }
 
 // iNTT of the own-slot registers v, centred lift, NTT: QP maps q -> p, else p -> q.
-// Leaves v in the other prime's transform domain (mod p canonical, mod q in [0, 4q)).
+// Leaves v in the other prime's transform domain (mod p in [0, 2p), mod q in [0, 4q)).
 template <int LOGN, bool QP, class Mid>
 __device__ __forceinline__ void round_trip(uint32_t v[8], uint32_t *X, const SmCtxDev &c, const Pair32 *twi,
                                            const Pair32 *twf, int t, Mid &&mid) {
