@@ -243,7 +243,9 @@ def sm_cells(rng, rows):
 def sm_cpu_baseline(budget_s):
     """The CPU oracle's special-modulus respond (C restatement of respond_impl) on a bounded
     sample of served-set queries against one prebuilt table and seat, on all of the box's
-    host cores (one query per thread) and on one."""
+    host cores (one query per thread) and on one; compiled for AVX-512F/BW/DQ/VL (the
+    reference's Kernel::Avx512) where the host has them, and portable. value is the all-core
+    rate of the faster form."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
@@ -253,16 +255,34 @@ def sm_cpu_baseline(budget_s):
     seat = ring.seat(rng.integers(0, oracle.SM_Q, SM_N, dtype=np.uint32),
                      rng.integers(0, oracle.SM_P, SM_N, dtype=np.uint32), bytes(32))
 
-    def one(r):
-        ring.respond(table, seat, r.bytes(32), r.integers(0, oracle.SM_Q, SM_N, dtype=np.uint32))
+    def runner(avx512):
+        def one(r):
+            ring.respond(table, seat, r.bytes(32), r.integers(0, oracle.SM_Q, SM_N, dtype=np.uint32), avx512=avx512)
+        return one
 
-    n1, dt1 = _timed_queries(one, budget_s / 2, 1)
     T = _cpu_threads()
-    nT, dtT = _timed_queries(one, budget_s / 2, T)
-    return {"value": round(nT / dtT, 3), "unit": "queries/s", "cores": T, "kind": "port",
-            "single_core": round(n1 / dt1, 3),
-            "sample": f"{nT} ring2048_cell256 queries against one prebuilt table (C restatement), "
-                      f"{T} threads for {dtT:.1f} s; and {n1} on 1 thread for {dt1:.1f} s"}
+    has_avx512 = oracle.sm_avx512_available()
+    legs = [("portable", False)] + ([("avx512", True)] if has_avx512 else [])
+    share = budget_s / (2 * len(legs))
+    res = {}
+    for name, flag in legs:
+        n1, dt1 = _timed_queries(runner(flag), share, 1)
+        nT, dtT = _timed_queries(runner(flag), share, T)
+        res[name] = (n1, dt1, nT, dtT)
+    best = max(res, key=lambda k: res[k][2] / res[k][3])
+    n1, dt1, nT, dtT = res[best]
+    pn1, pdt1, pnT, pdtT = res["portable"]
+    out = {"value": round(nT / dtT, 3), "unit": "queries/s", "cores": T, "kind": "port",
+           "single_core": round(n1 / dt1, 3), "simd": "avx512" if best == "avx512" else None,
+           "without_simd": {"value": round(pnT / pdtT, 3), "single_core": round(pn1 / pdt1, 3)},
+           "sample": f"ring2048_cell256 queries against one prebuilt table (C restatement), "
+                     f"{'compiled for AVX-512F/BW/DQ/VL' if best == 'avx512' else 'portable'}: "
+                     f"{nT} on {T} threads in {dtT:.1f} s and {n1} on 1 thread in {dt1:.1f} s; portable "
+                     f"{pnT} on {T} threads in {pdtT:.1f} s and {pn1} on 1 thread in {pdt1:.1f} s"}
+    if has_avx512:
+        an1, adt1, anT, adtT = res["avx512"]
+        out["avx512"] = {"value": round(anT / adtT, 3), "single_core": round(an1 / adt1, 3)}
+    return out
 
 
 def special_modulus(g, torch, dev, stream, args):

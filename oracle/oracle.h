@@ -238,6 +238,12 @@ int sm_seat(const sm_ring *r, const uint32_t *key_q, const uint32_t *key_p, cons
 int sm_respond(const sm_ring *r, const uint32_t *step1, const int8_t *values, size_t rows,
                const uint32_t *sq, const uint32_t *sp, const uint32_t *w_q, const uint32_t *w_p,
                const uint8_t seed[32], const uint32_t *qbody, uint32_t *mask, uint32_t *body, size_t nb);
+/* the same respond compiled for AVX-512F/BW/DQ/VL (special_avx512.c); call only where
+ * sm_avx512_available() returns 1 */
+int sm_respond_avx512(const sm_ring *r, const uint32_t *step1, const int8_t *values, size_t rows,
+                      const uint32_t *sq, const uint32_t *sp, const uint32_t *w_q, const uint32_t *w_p,
+                      const uint8_t seed[32], const uint32_t *qbody, uint32_t *mask, uint32_t *body, size_t nb);
+int sm_avx512_available(void);
 sm_client *sm_client_from_seed(const sm_ring *r, double sigma, const uint8_t seed[32]);
 void sm_client_free(sm_client *c);
 void sm_client_key(const sm_client *c, uint8_t mask_seed[32], uint32_t *key_q, uint32_t *key_p);

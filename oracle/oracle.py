@@ -269,6 +269,8 @@ _lib.sm_table.argtypes = [ctypes.POINTER(_SmRing), ctypes.c_size_t, ctypes.c_siz
 _lib.sm_seat.argtypes = [ctypes.POINTER(_SmRing), _u32p, _u32p, _u8p, _u32p, _u32p, _u32p, _u32p]
 _lib.sm_respond.argtypes = [ctypes.POINTER(_SmRing), _u32p, _i8p, ctypes.c_size_t, _u32p, _u32p, _u32p, _u32p,
                             _u8p, _u32p, _u32p, _u32p, ctypes.c_size_t]
+_lib.sm_respond_avx512.argtypes = _lib.sm_respond.argtypes
+_lib.sm_avx512_available.restype = ctypes.c_int
 _lib.sm_client_from_seed.argtypes = [ctypes.POINTER(_SmRing), ctypes.c_double, _u8p]
 _lib.sm_client_from_seed.restype = ctypes.c_void_p
 _lib.sm_client_free.argtypes = [ctypes.c_void_p]
@@ -306,6 +308,11 @@ def sm_kat_cells(seed, rows, worst=False, cell_bytes=SM_CELL, record_bytes=SM_RE
     out = np.zeros(rows * cell_bytes, dtype=np.uint8)
     _lib.sm_kat_cells(seed, rows, cell_bytes, record_bytes, int(worst), out.ctypes.data_as(_u8p))
     return out
+
+
+def sm_avx512_available():
+    """Kernel::Avx512's feature test (server.rs:50-58) on this CPU."""
+    return bool(_lib.sm_avx512_available())
 
 
 class SmRing:
@@ -384,14 +391,19 @@ class SmRing:
             raise ValueError("packing key coefficient is not a residue")
         return tuple(outs)  # (key_q, key_p, w_q, w_p), transform form, Montgomery operands
 
-    def respond(self, table, seat, seed, body):
+    def respond(self, table, seat, seed, body, avx512=False):
+        """respond_impl; avx512=True runs the same code compiled for AVX-512F/BW/DQ/VL, the
+        reference's Kernel::Avx512 (refused where the CPU lacks those features)."""
         step1, values, rows = table
         mask = np.zeros(self.d, np.uint32)
         out = np.zeros(self.nb, np.uint32)
         s, _k = _seed(seed)
         b = _u32(body, self.d)
-        if _lib.sm_respond(ctypes.byref(self._r), _p32(step1), values.ctypes.data_as(_i8p), rows,
-                           *[_p32(x) for x in seat], s, _p32(b), _p32(mask), _p32(out), self.nb) != 0:
+        if avx512 and not sm_avx512_available():
+            raise RuntimeError("this CPU lacks AVX-512F/BW/DQ/VL")
+        fn = _lib.sm_respond_avx512 if avx512 else _lib.sm_respond
+        if fn(ctypes.byref(self._r), _p32(step1), values.ctypes.data_as(_i8p), rows,
+              *[_p32(x) for x in seat], s, _p32(b), _p32(mask), _p32(out), self.nb) != 0:
             raise ValueError("query does not fit the set")
         return mask, out
 

@@ -11,7 +11,8 @@
  *                and mod_down)
  *   sm_table     Server::table + step1_row_impl, server.rs:211-250, 330-383
  *   sm_seat      Server::seat, server.rs:177-203
- *   sm_respond   respond_impl, server.rs:385-499
+ *   sm_respond   respond_impl, server.rs:385-499 (the respond path's arithmetic is in
+ *                sm_arith.h, shared with the AVX-512 build in special_avx512.c)
  *   sm_client_*  Client::from_seed / drawn / build / query / decode,
  *                client.rs:137-330 (stream "key seed" for the key-mask seed)
  *   sm_pack      the tight codec at a prime's bit width, wire.rs:10-21
@@ -23,6 +24,7 @@
  * (mod.rs:436-476) and decode of every record.
  */
 #include "oracle.h"
+#include "sm_arith.h"
 
 #include <stdlib.h>
 #include <string.h>
@@ -46,15 +48,10 @@ int sm_prime_init(sm_prime *p, uint64_t q) {
     return 0;
 }
 
-uint32_t sm_csub(const sm_prime *p, uint32_t x) { return x >= p->q ? x - p->q : x; }
-uint32_t sm_add(const sm_prime *p, uint32_t a, uint32_t b) { return sm_csub(p, a + b); }
-uint32_t sm_sub(const sm_prime *p, uint32_t a, uint32_t b) { return sm_csub(p, a + p->q - b); }
-
-uint32_t sm_mont(const sm_prime *p, uint32_t a, uint32_t b) {
-    uint64_t t = (uint64_t)a * b;
-    uint32_t m = (uint32_t)t * p->neg_qinv;
-    return sm_csub(p, (uint32_t)((t + (uint64_t)m * p->q) >> 32));
-}
+uint32_t sm_csub(const sm_prime *p, uint32_t x) { return smi_csub(p, x); }
+uint32_t sm_add(const sm_prime *p, uint32_t a, uint32_t b) { return smi_add(p, a, b); }
+uint32_t sm_sub(const sm_prime *p, uint32_t a, uint32_t b) { return smi_sub(p, a, b); }
+uint32_t sm_mont(const sm_prime *p, uint32_t a, uint32_t b) { return smi_mont(p, a, b); }
 
 uint32_t sm_to_mont(const sm_prime *p, uint32_t a) { return sm_mont(p, a, p->r2); }
 
@@ -63,7 +60,7 @@ uint32_t sm_small(const sm_prime *p, int64_t x) {
     return (uint32_t)(r < 0 ? r + p->q : r);
 }
 
-int64_t sm_centred(const sm_prime *p, uint32_t x) { return x > p->q / 2 ? (int64_t)x - p->q : (int64_t)x; }
+int64_t sm_centred(const sm_prime *p, uint32_t x) { return smi_centred(p, x); }
 
 uint32_t sm_pow(const sm_prime *p, uint64_t base, uint64_t e) {
     uint64_t acc = 1, b = base % p->q;
@@ -126,45 +123,8 @@ void sm_ntt_free(sm_ntt *c) {
     c->fwd = c->inv = NULL;
 }
 
-void sm_ntt_forward(const sm_ntt *c, uint32_t *a) {
-    const sm_prime *p = &c->p;
-    size_t n = c->n, t = n;
-    for (size_t m = 1; m < n; m <<= 1) {
-        t >>= 1;
-        for (size_t blk = 0; blk < m; blk++) {
-            uint32_t s = c->fwd[m + blk];
-            uint32_t *x = a + 2 * blk * t, *y = x + t;
-            for (size_t j = 0; j < t; j++) {
-                uint32_t u = x[j], v = sm_mont(p, y[j], s);
-                x[j] = sm_add(p, u, v);
-                y[j] = sm_sub(p, u, v);
-            }
-        }
-    }
-}
-
-void sm_ntt_inverse(const sm_ntt *c, uint32_t *a) {
-    const sm_prime *p = &c->p;
-    size_t n = c->n, t = 1;
-    for (size_t m = n; m > 2; m >>= 1) {
-        size_t h = m >> 1;
-        for (size_t blk = 0; blk < h; blk++) {
-            uint32_t s = c->inv[h + blk];
-            uint32_t *x = a + 2 * blk * t, *y = x + t;
-            for (size_t j = 0; j < t; j++) {
-                uint32_t u = x[j], v = y[j];
-                x[j] = sm_add(p, u, v);
-                y[j] = sm_mont(p, u + p->q - v, s);
-            }
-        }
-        t <<= 1;
-    }
-    for (size_t j = 0; j < t; j++) {
-        uint32_t u = a[j], v = a[j + t];
-        a[j] = sm_mont(p, u + v, c->n_inv);
-        a[j + t] = sm_mont(p, u + p->q - v, c->last);
-    }
-}
+void sm_ntt_forward(const sm_ntt *c, uint32_t *a) { smi_ntt_forward(c, a); }
+void sm_ntt_inverse(const sm_ntt *c, uint32_t *a) { smi_ntt_inverse(c, a); }
 
 typedef struct { uint32_t v; uint16_t slot; } sm_vs;
 
@@ -273,21 +233,9 @@ void sm_key_mask(const sm_ring *r, const uint8_t seed[32], uint32_t *w_q, uint32
     }
 }
 
-uint32_t sm_p_to_q_centred(const sm_ring *r, uint32_t x) {
-    const sm_prime *pq = &r->lq.p;
-    uint32_t base = x;
-    for (uint32_t k = 0; k < r->p_over_q; k++) base = sm_csub(pq, base);
-    return x > r->lp.p.q / 2 ? sm_add(pq, base, r->neg_off) : base;
-}
-
-uint32_t sm_q_to_p_centred(const sm_ring *r, uint32_t c) {
-    return c > r->lq.p.q / 2 ? c + (r->lp.p.q - r->lq.p.q) : c;
-}
-
-uint32_t sm_mod_down(const sm_ring *r, uint32_t xq, uint32_t xp) {
-    const sm_prime *pq = &r->lq.p;
-    return sm_mont(pq, sm_sub(pq, xq, sm_p_to_q_centred(r, xp)), r->p_inv);
-}
+uint32_t sm_p_to_q_centred(const sm_ring *r, uint32_t x) { return smi_p_to_q_centred(r, x); }
+uint32_t sm_q_to_p_centred(const sm_ring *r, uint32_t c) { return smi_q_to_p_centred(r, c); }
+uint32_t sm_mod_down(const sm_ring *r, uint32_t xq, uint32_t xp) { return smi_mod_down(r, xq, xp); }
 
 /* ---------------- Server ---------------- */
 
@@ -376,66 +324,19 @@ int sm_seat(const sm_ring *r, const uint32_t *key_q, const uint32_t *key_p, cons
     return 0;
 }
 
-/* respond_impl (server.rs:385-499). nb body coefficients are returned. */
+/* respond_impl (server.rs:385-499), portable build. nb body coefficients are returned. */
 int sm_respond(const sm_ring *r, const uint32_t *step1, const int8_t *values, size_t rows,
                const uint32_t *sq, const uint32_t *sp, const uint32_t *w_q, const uint32_t *w_p,
                const uint8_t seed[32], const uint32_t *qbody, uint32_t *mask, uint32_t *body, size_t nb) {
-    size_t d = r->d, gamma = r->gamma;
-    const sm_ntt *lq = &r->lq, *lp = &r->lp;
-    const sm_prime *pq = &lq->p, *pp = &lp->p;
-    for (size_t k = 0; k < d; k++)
-        if (qbody[k] >= pq->q) return -1;
+    return smi_respond(r, step1, values, rows, sq, sp, w_q, w_p, seed, qbody, mask, body, nb);
+}
 
-    int64_t *fold = (int64_t *)calloc(nb, 8);
-    for (size_t m = 0; m < rows; m++) {
-        int64_t c = m == 0 ? sm_centred(pq, qbody[0]) : -sm_centred(pq, qbody[d - m]);
-        const int8_t *row = values + m * gamma;
-        for (size_t k = 0; k < nb; k++) fold[k] += c * row[k];
-    }
-
-    uint32_t *a = (uint32_t *)malloc(d * 4), *cur = (uint32_t *)malloc(d * 4),
-             *next = (uint32_t *)malloc(d * 4), *tp = (uint32_t *)malloc(d * 4),
-             *cp = (uint32_t *)malloc(d * 4), *rq = (uint32_t *)malloc(d * 4),
-             *acc_q = (uint32_t *)calloc(d, 4), *acc_p = (uint32_t *)calloc(d, 4);
-    sm_expand_query_mask(r, seed, a);
-    sm_ntt_forward(lq, a);
-
-    {
-        const uint16_t *tab = r->tables + (gamma - 1) * d;
-        const uint32_t *s = step1 + (gamma - 1) * d;
-        for (size_t j = 0; j < d; j++) cur[j] = sm_mont(pq, a[tab[j]], s[j]);
-    }
-    for (size_t ii = gamma - 1; ii-- > 0;) {
-        const uint16_t *tab = r->tables + ii * d;
-        memcpy(tp, cur, d * 4);
-        sm_ntt_inverse(lq, tp);
-        for (size_t j = 0; j < d; j++) tp[j] = sm_q_to_p_centred(r, tp[j]);
-        sm_ntt_forward(lp, tp);
-        for (size_t j = 0; j < d; j++) cp[j] = sm_mont(pp, tp[j], w_p[tab[j]]);
-        sm_ntt_inverse(lp, cp);
-        for (size_t j = 0; j < d; j++) rq[j] = sm_p_to_q_centred(r, cp[j]);
-        sm_ntt_forward(lq, rq);
-        const uint32_t *s = step1 + ii * d;
-        for (size_t j = 0; j < d; j++) {
-            size_t src = tab[j];
-            uint32_t cq = sm_mont(pq, cur[j], w_q[src]);
-            next[j] = sm_add(pq, sm_mont(pq, a[src], s[j]), sm_mont(pq, sm_sub(pq, cq, rq[j]), r->p_inv));
-            acc_q[j] = sm_add(pq, acc_q[j], sm_mont(pq, cur[j], sq[src]));
-            acc_p[j] = sm_add(pp, acc_p[j], sm_mont(pp, tp[j], sp[src]));
-        }
-        uint32_t *t = cur; cur = next; next = t;
-    }
-    memcpy(mask, cur, d * 4);
-    sm_ntt_inverse(lq, mask);
-    sm_ntt_inverse(lq, acc_q);
-    sm_ntt_inverse(lp, acc_p);
-    for (size_t k = 0; k < nb; k++) {
-        int64_t f = fold[k] % (int64_t)pq->q;
-        if (f < 0) f += pq->q;
-        body[k] = sm_add(pq, sm_mod_down(r, acc_q[k], acc_p[k]), (uint32_t)f);
-    }
-    free(fold); free(a); free(cur); free(next); free(tp); free(cp); free(rq); free(acc_q); free(acc_p);
-    return 0;
+/* Kernel::Avx512's availability test (server.rs:50-58): the respond_avx512 form runs
+ * only where the CPU has all four feature sets. */
+int sm_avx512_available(void) {
+    __builtin_cpu_init();
+    return __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512bw") &&
+           __builtin_cpu_supports("avx512dq") && __builtin_cpu_supports("avx512vl");
 }
 
 /* ---------------- Client ---------------- */

@@ -20,7 +20,14 @@ def served():
     return oracle.SmRing(cell_bytes=256, record_bytes=246)
 
 
-def test_served_wire_bytes_are_pinned(served):
+# respond_portable and respond_avx512 (server.rs:306-330): the same respond_impl, the second
+# compiled for AVX-512F/BW/DQ/VL
+FORMS = ["portable", pytest.param("avx512", marks=pytest.mark.skipif(
+    not oracle.sm_avx512_available(), reason="this CPU lacks AVX-512F/BW/DQ/VL"))]
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_served_wire_bytes_are_pinned(served, form):
     client = oracle.SmClient(served, bytes([42] * 32))
     cells = oracle.sm_kat_cells(42, 2048)
     table = served.table(cells)
@@ -28,12 +35,30 @@ def test_served_wire_bytes_are_pinned(served):
     seat = served.seat(kq, kp, seed)
     key = client.key_bytes()
     qseed, qbody = client.query(1234)
-    mask, body = served.respond(table, seat, qseed, qbody)
+    mask, body = served.respond(table, seat, qseed, qbody, avx512=form == "avx512")
     query = qseed + oracle.sm_pack(qbody, SM_Q)
     response = served.response_bytes(mask, body)
     assert (len(key), len(query), len(response)) == (15_394, 7_456, 9_208)
     assert [oracle.fnv1a(key), oracle.fnv1a(query), oracle.fnv1a(response)] == PINNED
     assert client.decode(mask, body) == bytes(cells[1234 * 256:1234 * 256 + 246])
+
+
+@pytest.mark.skipif(not oracle.sm_avx512_available(), reason="this CPU lacks AVX-512F/BW/DQ/VL")
+@pytest.mark.parametrize("d,gamma", [(2048, 512), (256, 32), (64, 8)])
+def test_avx512_build_equals_the_portable_build(d, gamma):
+    ring = oracle.SmRing(cell_bytes=256, record_bytes=246) if d == 2048 else oracle.SmRing(d=d, gamma=gamma)
+    rng = np.random.default_rng(d)
+    rows, cb = min(d, 300), ring.cell_bytes
+    cells = np.frombuffer(rng.bytes(rows * cb), dtype=np.uint8).reshape(rows, cb).copy()
+    cells[:, ring.record_bytes:] = 0
+    table = ring.table(cells.tobytes())
+    seat = ring.seat(rng.integers(0, SM_Q, d, dtype=np.uint32), rng.integers(0, SM_P, d, dtype=np.uint32),
+                     rng.bytes(32))
+    for _ in range(2):
+        seed, body = rng.bytes(32), rng.integers(0, SM_Q, d, dtype=np.uint32)
+        m0, b0 = ring.respond(table, seat, seed, body)
+        m1, b1 = ring.respond(table, seat, seed, body, avx512=True)
+        assert np.array_equal(m0, m1) and np.array_equal(b0, b1)
 
 
 def _naive_negacyclic(a, b, m):
