@@ -170,6 +170,31 @@ def test_refusals(gpu):
         gpu.SmServer(bad_params)
 
 
+def test_empty_and_oversize_batches(gpu):
+    """Through the C ABI: an empty batch answers nothing on all four batch entry points (the
+    loop over no queries), and a count past INSPIRE_MAX_BATCH is refused before any of the
+    batch's arrays is read (the placeholder address 1 is never dereferenced)."""
+    import ctypes
+
+    L = gpu.lib()
+    with gpu.SmServer(gpu.ring2048_cell256()) as server:
+        h, z = server._h, None
+        gpu._call(L.inspire_sm_respond_batch, h, 0, z, z, z, z, z, z)
+        gpu._call(L.inspire_sm_respond_batch_wire, h, 0, z, z, z, z, z, 0)
+        gpu._call(L.inspire_sm_respond_batch_device, h, 0, z, z, z, z, z, z, z, z)
+        gpu._call(L.inspire_sm_respond_batch_device_wire, h, 0, z, z, z, z, z, z, 0, z)
+        x, big = ctypes.c_void_p(1), (1 << 22) + 1
+        calls = [(L.inspire_sm_respond_batch, (x,) * 6),
+                 (L.inspire_sm_respond_batch_wire, (x,) * 5 + (9216,)),
+                 (L.inspire_sm_respond_batch_device, (x,) * 6 + (z, z)),
+                 (L.inspire_sm_respond_batch_device_wire, (x,) * 6 + (9216, z))]
+        for fn, args in calls:
+            with pytest.raises(gpu.PirError, match="a batch holds at most 4194304 queries, got 4194305"):
+                gpu._call(fn, h, big, *args)
+        with pytest.raises(gpu.PirError, match="null argument"):
+            gpu._call(L.inspire_sm_respond_batch, h, 1, z, z, z, z, z, z)
+
+
 @pytest.mark.parametrize("d,gamma", [(2048, 512), (256, 32), (64, 8)])
 def test_device_wire_bytes_equal_the_host_codec(gpu, d, gamma):
     """Response::to_bytes packed by the kernel equals the host codec over the word arrays of
