@@ -128,8 +128,9 @@ int inspire_workspace_create(inspire_ctx *ctx, size_t max_count, inspire_workspa
 void inspire_workspace_destroy(inspire_workspace *ws);
 /* inspire_respond_batch with every buffer in device memory, enqueued on `stream`
  * (a hipStream_t, NULL = default stream) and not waited for. Every shard named must
- * already hold a store. Per-query status lands in the workspace: read it with
- * inspire_workspace_status after the stream completes. */
+ * already hold a store; the ids are read on the device, and a query whose id names no shard
+ * with a store gets status -1 and no output. Per-query status lands in the workspace: read
+ * it with inspire_workspace_status after the stream completes. */
 int inspire_respond_batch_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws,
                                  const uint32_t *d_shard_ids, const uint8_t *d_seeds, const uint64_t *d_b,
                                  size_t count, uint64_t switch_modulus, uint64_t *d_a_out,
@@ -142,7 +143,17 @@ int inspire_respond_batch_device_timed(inspire_ctx *ctx, const inspire_seat *sea
                                        size_t count, uint64_t switch_modulus, uint64_t *d_a_out,
                                        uint64_t *d_b_prefix_out, void *stream, float kernel_ms[2], char *err,
                                        size_t err_cap);
-/* Copies count statuses (0 = answered, c+1 = column c failed its store tag). */
+/* Kernel trace (measurement; no counterpart in the reference): arms ws for its next
+ * `launches` device-buffer launches (at most 2^20; 0 disarms), each of which then records HIP
+ * events on its own stream around the DB-scan and recursion kernels without waiting, so a
+ * timed loop keeps its pipelining. inspire_workspace_kernel_times waits for the last traced
+ * launch, writes the durations (ms) of up to cap of them to scan_ms and chain_ms, their
+ * number to *count, and disarms. Like every use of a workspace, one caller at a time. */
+int inspire_workspace_trace(inspire_workspace *ws, size_t launches, char *err, size_t err_cap);
+int inspire_workspace_kernel_times(inspire_workspace *ws, float *scan_ms, float *chain_ms, size_t cap,
+                                   size_t *count, char *err, size_t err_cap);
+/* Copies count statuses (0 = answered, c+1 = column c failed its store tag, -1 = the id names
+ * no shard with a store). */
 int inspire_workspace_status(inspire_workspace *ws, size_t count, int32_t *status, char *err, size_t err_cap);
 
 /* ServerResponse::to_binary bytes straight from the device (respond.rs:387, the bincode
@@ -290,6 +301,14 @@ int inspire_sm_respond_batch_device_wire(inspire_sm_server *server, size_t count
                                          const uint8_t *d_seeds, const uint32_t *d_bodies, uint32_t *d_scratch,
                                          uint8_t *d_wire_out, size_t wire_stride, void *stream, char *err,
                                          size_t err_cap);
+/* Kernel trace (measurement; no counterpart in the reference): the server's next `launches`
+ * device-buffer launches (the two entry points above; at most 2^20, 0 disarms) record HIP
+ * events around the respond kernel on their own streams without waiting;
+ * inspire_sm_server_kernel_times waits for the last one, writes up to cap durations (ms) and
+ * their number, and disarms. */
+int inspire_sm_server_trace(inspire_sm_server *server, size_t launches, char *err, size_t err_cap);
+int inspire_sm_server_kernel_times(inspire_sm_server *server, float *kernel_ms, size_t cap, size_t *count,
+                                   char *err, size_t err_cap);
 
 /* Continuous-batching queue over one special-modulus server: requests (a batch of
  * queries under one seated key, each query naming its own table) from any thread are

@@ -450,12 +450,23 @@ static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_wo
         set_err(err, err_cap, "null argument");
         return INSPIRE_ERR_INVALID;
     }
+    hipStream_t st = static_cast<hipStream_t>(stream);
     if (!timing) {
-        HIP_TRY(launch_respond(dev, b, static_cast<hipStream_t>(stream)));
+        if (hipEvent_t *tr = ws->trace.next()) {
+            // a traced launch: events around both kernels, read later, no wait
+            hipError_t e = hipEventRecord(tr[0], st);
+            if (e == hipSuccess) e = launch_scan_only(dev, b, st);
+            if (e == hipSuccess) e = hipEventRecord(tr[1], st);
+            if (e == hipSuccess) e = launch_chain_only(dev, b, st);
+            if (e == hipSuccess) e = hipEventRecord(tr[2], st);
+            HIP_TRY(e);
+            ws->trace.done();
+            return INSPIRE_OK;
+        }
+        HIP_TRY(launch_respond(dev, b, st));
         return INSPIRE_OK;
     }
     // per-kernel durations from events on the launch stream
-    hipStream_t st = static_cast<hipStream_t>(stream);
     hipEvent_t ev[3];
     for (int i = 0; i < 3; i++) HIP_TRY(hipEventCreate(&ev[i]));
     hipError_t e = hipEventRecord(ev[0], st);
@@ -477,6 +488,24 @@ int inspire_respond_batch_device(inspire_ctx *ctx, const inspire_seat *seat, ins
                                  void *stream, char *err, size_t err_cap) {
     return respond_device(ctx, seat, ws, d_shard_ids, d_seeds, d_b, count, switch_modulus, d_a_out, d_b_prefix_out,
                           stream, nullptr, err, err_cap);
+}
+
+int inspire_workspace_trace(inspire_workspace *ws, size_t launches, char *err, size_t err_cap) {
+    if (!ws) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    if (launches > (1u << 20)) {
+        set_err(err, err_cap, "a trace holds at most %u launches, asked for %zu", 1u << 20, launches);
+        return INSPIRE_ERR_INVALID;
+    }
+    HIP_TRY(ws->trace.arm(launches, 2));
+    return INSPIRE_OK;
+}
+
+int inspire_workspace_kernel_times(inspire_workspace *ws, float *scan_ms, float *chain_ms, size_t cap,
+                                   size_t *count, char *err, size_t err_cap) {
+    if (!ws || !count || (cap && (!scan_ms || !chain_ms))) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    float *const out[2] = {scan_ms, chain_ms};
+    HIP_TRY(ws->trace.read(out, 2, cap, count));
+    return INSPIRE_OK;
 }
 
 int inspire_respond_batch_device_timed(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws,
@@ -753,7 +782,10 @@ static int respond_host(inspire_ctx *ctx, const inspire_seat *seat, const uint32
     std::vector<int32_t> st(count);
     HIP_TRY(hipMemcpy(st.data(), ws->status.p, count * 4, hipMemcpyDeviceToHost));
     for (size_t k = 0; k < count; k++)
-        if (st[k] != 0) {
+        if (st[k] < 0) {
+            set_err(err, err_cap, "shard %u has no packing store on the device", shard_ids[k]);
+            return INSPIRE_ERR_STORE;
+        } else if (st[k] != 0) {
             set_err(err, err_cap,
                     "shard %u: packing store refused: column %d is not the column the store was built from. The "
                     "shard changed after the build; rebuild the store before serving it",

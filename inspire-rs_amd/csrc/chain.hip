@@ -320,7 +320,14 @@ void k_chain(CtxDev c, BatchDev bt) {
     auto aix = [](int e) { return e; };
 
     const int qi = blockIdx.x, tid = threadIdx.x, gamma = c.gamma;
-    const ShardDev sh = c.shards[bt.shard_ids[qi]];
+    // the ids of device-buffer batches are not checked on the host: a query naming no shard
+    // with a store is refused here, before anything reads the shard (the whole workgroup)
+    const uint32_t sid = bt.shard_ids[qi];
+    if (sid >= (uint32_t)c.num_shards || !c.shards[sid].store) {
+        if (tid == 0) bt.status[qi] = -1;
+        return;
+    }
+    const ShardDev sh = c.shards[sid];
     const Pair *twf = c.twf, *twi = c.twi;
     const Pair *Y = bt.yprimes ? bt.yprimes[qi] : bt.yprime;
     // Twiddles of the passes at base bit BL and up (ladder entries below n >> BL: 16 KiB at

@@ -50,7 +50,7 @@ struct BatchDev {
     int ks;                     // row splits of the scan
     uint64_t *a_out;            // count x n
     uint64_t *b_out;            // count x gamma
-    int32_t *status;            // count: 0 ok, c+1 = column c failed its tag
+    int32_t *status;            // count: 0 ok, c+1 = column c failed its tag, -1 = no shard with a store
     uint64_t switch_modulus;    // 0 = no modulus switch
     // when non-null, each query's ServerResponse bincode bytes at wire_out + qi * wire_stride
     // (4-byte aligned, stride a multiple of 4 holding the length rounded up to 4); a_out and

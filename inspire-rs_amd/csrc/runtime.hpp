@@ -16,6 +16,7 @@
 #include "device.hpp"
 #include "kernels.hpp"
 #include "modq.hpp"
+#include "trace.hpp"
 
 using namespace inspire;
 typedef unsigned __int128 u128;
@@ -118,6 +119,7 @@ struct inspire_workspace {
     // staging for the host-buffer entry point
     DevBuf ids, seeds, a_in, b, a_out, b_out, wire;
     size_t wire_cap = 0;
+    KernelTrace trace;  // scan and chain events of the next device-buffer launches, when armed
 };
 
 struct inspire_ctx {

@@ -194,7 +194,10 @@ void inspire_server::complete(Slot &s) {
             res.err = std::string("HIP failure completing a launch: ") + hipGetErrorString(e);
         } else {
             for (size_t k = 0; k < c && res.rc == INSPIRE_OK; k++)
-                if (s.h_status[q + k] != 0) {
+                if (s.h_status[q + k] < 0) {
+                    res.rc = INSPIRE_ERR_STORE;
+                    res.err = "shard " + std::to_string(r.ids[k]) + " has no packing store on the device";
+                } else if (s.h_status[q + k] != 0) {
                     char buf[320];
                     snprintf(buf, sizeof buf,
                              "shard %u: packing store refused: column %d is not the column the store was built "

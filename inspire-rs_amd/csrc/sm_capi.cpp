@@ -317,6 +317,17 @@ int check_handles(inspire_sm_server *s, size_t count, const inspire_sm_table *co
     return INSPIRE_OK;
 }
 
+// a device-buffer launch, with events around its kernel while a trace is armed (no wait)
+hipError_t launch_traced(inspire_sm_server *s, const SmBatchDev &b, hipStream_t st) {
+    hipEvent_t *tr = s->trace.next();
+    if (!tr) return sm_launch_respond(s->dev, b, st);
+    hipError_t e = hipEventRecord(tr[0], st);
+    if (e == hipSuccess) e = sm_launch_respond(s->dev, b, st);
+    if (e == hipSuccess) e = hipEventRecord(tr[1], st);
+    if (e == hipSuccess) s->trace.done();
+    return e;
+}
+
 }  // namespace
 
 extern "C" {
@@ -468,7 +479,7 @@ int inspire_sm_respond_batch_device_wire(inspire_sm_server *s, size_t count, con
     SmBatchDev b{(int)count, s->djobs.as<SmJob>(), d_seeds, d_bodies, d_scratch, nullptr};
     b.wire_out = d_wire_out;
     b.wire_stride = wire_stride;
-    SM_TRY(sm_launch_respond(s->dev, b, st));
+    SM_TRY(launch_traced(s, b, st));
     return INSPIRE_OK;
 }
 
@@ -494,7 +505,7 @@ int inspire_sm_respond_batch_device(inspire_sm_server *s, size_t count, const in
     if (rc) return rc;
     SmBatchDev b{(int)count, s->djobs.as<SmJob>(), d_seeds, d_bodies, d_mask_out, d_body_out};
     if (!kernel_ms) {
-        SM_TRY(sm_launch_respond(s->dev, b, st));
+        SM_TRY(launch_traced(s, b, st));
         return INSPIRE_OK;
     }
     hipEvent_t ev[2];
@@ -508,6 +519,32 @@ int inspire_sm_respond_batch_device(inspire_sm_server *s, size_t count, const in
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
     SM_TRY(e);
+    return INSPIRE_OK;
+}
+
+int inspire_sm_server_trace(inspire_sm_server *s, size_t launches, char *err, size_t err_cap) {
+    if (!s) {
+        sm_err(err, err_cap, "null argument");
+        return INSPIRE_ERR_INVALID;
+    }
+    if (launches > (1u << 20)) {
+        sm_err(err, err_cap, "a trace holds at most %u launches, asked for %zu", 1u << 20, launches);
+        return INSPIRE_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    SM_TRY(s->trace.arm(launches, 1));
+    return INSPIRE_OK;
+}
+
+int inspire_sm_server_kernel_times(inspire_sm_server *s, float *kernel_ms, size_t cap, size_t *count, char *err,
+                                   size_t err_cap) {
+    if (!s || !count || (cap && !kernel_ms)) {
+        sm_err(err, err_cap, "null argument");
+        return INSPIRE_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    float *const out[1] = {kernel_ms};
+    SM_TRY(s->trace.read(out, 1, cap, count));
     return INSPIRE_OK;
 }
 

@@ -10,6 +10,7 @@
 
 #include "../../include/inspire_gpu.h"
 #include "sm.hpp"
+#include "trace.hpp"
 
 namespace inspire_sm {
 
@@ -61,6 +62,8 @@ struct inspire_sm_server {
     size_t wire_cap = 0;
     // the host-buffer entry points launch here and wait for this stream alone
     hipStream_t stream = nullptr;
+    // kernel events of the next device-buffer launches, when armed
+    inspire::KernelTrace trace;
     ~inspire_sm_server() {
         if (stream) {
             (void)hipStreamSynchronize(stream);

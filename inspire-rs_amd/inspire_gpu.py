@@ -86,6 +86,8 @@ def lib():
         L.inspire_workspace_destroy.argtypes = [vp]
         L.inspire_respond_batch_device.argtypes = [vp, vp, vp, vp, vp, vp, sz, u64, vp, vp, vp, cp, sz]
         L.inspire_respond_batch_device_timed.argtypes = [vp, vp, vp, vp, vp, vp, sz, u64, vp, vp, vp, vp, cp, sz]
+        L.inspire_workspace_trace.argtypes = [vp, sz, cp, sz]
+        L.inspire_workspace_kernel_times.argtypes = [vp, vp, vp, sz, vp, cp, sz]
         L.inspire_server_create.argtypes = [vp, sz, sz, pp, cp, sz]
         L.inspire_server_destroy.argtypes = [vp]
         L.inspire_server_register_seat.argtypes = [vp, vp, u32, u32, ctypes.POINTER(u64), cp, sz]
@@ -123,6 +125,8 @@ def lib():
         L.inspire_sm_response_wire_bytes.restype = sz
         L.inspire_sm_respond_batch_wire.argtypes = [vp, sz, vp, vp, vp, vp, vp, sz, cp, sz]
         L.inspire_sm_respond_batch_device_wire.argtypes = [vp, sz, vp, vp, vp, vp, vp, vp, sz, vp, cp, sz]
+        L.inspire_sm_server_trace.argtypes = [vp, sz, cp, sz]
+        L.inspire_sm_server_kernel_times.argtypes = [vp, vp, sz, vp, cp, sz]
         L.inspire_sm_batcher_create.argtypes = [vp, sz, sz, pp, cp, sz]
         L.inspire_sm_batcher_destroy.argtypes = [vp]
         L.inspire_sm_batcher_register_key.argtypes = [vp, vp, vp, vp, ctypes.POINTER(u64), cp, sz]
@@ -190,6 +194,20 @@ class Workspace:
         out = np.zeros(count, dtype=np.int32)
         _call(lib().inspire_workspace_status, self._h, count, _ptr(out))
         return out
+
+    def trace(self, launches):
+        """Record scan and chain kernel events on the next `launches` device launches."""
+        _call(lib().inspire_workspace_trace, self._h, launches)
+        self._traced = launches
+
+    def kernel_times(self):
+        """(scan_ms, chain_ms) arrays of the traced launches; waits for the last one."""
+        cap = getattr(self, "_traced", 0)
+        scan, chain = np.zeros(cap, np.float32), np.zeros(cap, np.float32)
+        n = ctypes.c_size_t()
+        _call(lib().inspire_workspace_kernel_times, self._h, _ptr(scan), _ptr(chain), cap, ctypes.byref(n))
+        self._traced = 0
+        return scan[:n.value], chain[:n.value]
 
     def close(self):
         if self._h:
@@ -729,6 +747,20 @@ class SmServer:
         _call(lib().inspire_sm_respond_batch_device_wire, self._h, count, th, sh, ctypes.c_void_p(d_seeds),
               ctypes.c_void_p(d_bodies), ctypes.c_void_p(d_scratch), ctypes.c_void_p(d_wire), wire_stride,
               ctypes.c_void_p(stream))
+
+    def trace(self, launches):
+        """Record respond-kernel events on the next `launches` device launches."""
+        _call(lib().inspire_sm_server_trace, self._h, launches)
+        self._traced = launches
+
+    def kernel_times(self):
+        """Respond-kernel durations (ms) of the traced launches; waits for the last one."""
+        cap = getattr(self, "_traced", 0)
+        out = np.zeros(cap, np.float32)
+        n = ctypes.c_size_t()
+        _call(lib().inspire_sm_server_kernel_times, self._h, _ptr(out), cap, ctypes.byref(n))
+        self._traced = 0
+        return out[:n.value]
 
     def respond_batch_device(self, tables, seats, d_seeds, d_bodies, d_mask, d_body, stream=0, timed=False):
         """Device-resident inputs and outputs (addresses); returns the kernel ms when timed."""

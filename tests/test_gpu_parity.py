@@ -173,3 +173,76 @@ def test_empty_batch(gpu):
         ws.close()
         foreign.close()
         seat.close()
+
+
+def test_kernel_trace_keeps_results(gpu):
+    """Launches traced by the workspace (events around the scan and chain kernels, read after
+    the loop) answer exactly as untraced ones; only the armed number of launches is recorded."""
+    import torch
+
+    n, gamma, L = 2048, 16, 8
+    rng = np.random.default_rng(5)
+    dev = torch.device("cuda:0")
+    with gpu.Context(gpu.params_d2048(), bytes(32), gamma) as ctx:
+        ctx.upload_columns(0, rng.integers(0, 1 << 16, (gamma, n), dtype=np.uint64)).precompute_packing()
+        seat = ctx.seat(rng.integers(0, gpu.Q, (3, n), dtype=np.uint64))
+        ws = gpu.Workspace(ctx, L)
+        ids = torch.zeros(L, dtype=torch.int32, device=dev)
+        seeds = torch.from_numpy(rng.integers(0, 256, (L, 32), dtype=np.uint8)).to(dev)
+        b = torch.from_numpy(rng.integers(0, gpu.Q, (L, n), dtype=np.uint64).view(np.int64)).to(dev)
+        outs = []
+        for traced in (False, True):
+            a_out = torch.empty((L, n), dtype=torch.int64, device=dev)
+            b_out = torch.empty((L, gamma), dtype=torch.int64, device=dev)
+            if traced:
+                ws.trace(2)
+            for _ in range(3):
+                ctx.respond_batch_device(seat, ws, ids.data_ptr(), seeds.data_ptr(), b.data_ptr(), L,
+                                         a_out.data_ptr(), b_out.data_ptr())
+            if traced:
+                scan, chain = ws.kernel_times()
+                assert len(scan) == len(chain) == 2 and (scan > 0).all() and (chain > 0).all()
+            torch.cuda.synchronize()
+            assert (ws.status(L) == 0).all()
+            outs.append((a_out.cpu(), b_out.cpu()))
+        assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
+        with pytest.raises(gpu.PirError, match="at most 1048576 launches"):
+            ws.trace((1 << 20) + 1)
+        ws.close()
+        seat.close()
+
+
+def test_device_ids_without_a_store_are_refused(gpu):
+    """The ids of a device-buffer batch are read on the device: a query naming no shard with a
+    store (an id past the shard table, or a shard whose store was never built) gets status -1
+    and nothing reads that shard; the other queries of the batch are answered as the host
+    path answers them."""
+    import torch
+
+    n, gamma = 2048, 16
+    rng = np.random.default_rng(8)
+    dev = torch.device("cuda:0")
+    with gpu.Context(gpu.params_d2048(), bytes(32), gamma) as ctx:
+        ctx.upload_columns(0, rng.integers(0, 1 << 16, (gamma, n), dtype=np.uint64)).precompute_packing()
+        ctx.upload_columns(1, rng.integers(0, 1 << 16, (gamma, n), dtype=np.uint64))
+        seat = ctx.seat(rng.integers(0, gpu.Q, (3, n), dtype=np.uint64))
+        ids = [0, 1, 7, 0]
+        seeds = rng.integers(0, 256, (len(ids), 32), dtype=np.uint8)
+        bs = rng.integers(0, gpu.Q, (len(ids), n), dtype=np.uint64)
+        ws = gpu.Workspace(ctx, len(ids))
+        d_ids = torch.tensor(ids, dtype=torch.int32, device=dev)
+        d_seeds = torch.from_numpy(seeds).to(dev)
+        d_b = torch.from_numpy(bs.view(np.int64)).to(dev)
+        a_out = torch.zeros((len(ids), n), dtype=torch.int64, device=dev)
+        b_out = torch.zeros((len(ids), gamma), dtype=torch.int64, device=dev)
+        print("device batch over shard ids", ids, flush=True)
+        ctx.respond_batch_device(seat, ws, d_ids.data_ptr(), d_seeds.data_ptr(), d_b.data_ptr(), len(ids),
+                                 a_out.data_ptr(), b_out.data_ptr())
+        torch.cuda.synchronize()
+        assert list(ws.status(len(ids))) == [0, -1, -1, 0]
+        A, B = ctx.respond_seeded_batch(seat, [0, 0], [seeds[0].tobytes(), seeds[3].tobytes()], bs[[0, 3]])
+        got_a, got_b = a_out.cpu().numpy().view(np.uint64), b_out.cpu().numpy().view(np.uint64)
+        assert np.array_equal(got_a[[0, 3]], A) and np.array_equal(got_b[[0, 3]], B)
+        ws.close()
+        seat.close()
+

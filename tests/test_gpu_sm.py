@@ -195,6 +195,40 @@ def test_empty_and_oversize_batches(gpu):
             gpu._call(L.inspire_sm_respond_batch, h, 1, z, z, z, z, z, z)
 
 
+def test_kernel_trace_keeps_results(gpu):
+    """Launches traced by the server (events around the respond kernel, read after the loop)
+    answer exactly as untraced ones; only the armed number of launches is recorded."""
+    import torch
+
+    rng = np.random.default_rng(6)
+    dev = torch.device("cuda:0")
+    L, n = 4, 2048
+    with gpu.SmServer(gpu.ring2048_cell256()) as server:
+        table = server.table(oracle.sm_kat_cells(5, 64).tobytes())
+        seat = server.seat(_random_key(gpu, rng, n))
+        seeds = torch.from_numpy(rng.integers(0, 256, (L, 32), dtype=np.uint8)).to(dev)
+        bodies = torch.from_numpy(rng.integers(0, SM_Q, (L, n), dtype=np.int64).astype(np.int32)).to(dev)
+        outs = []
+        for traced in (False, True):
+            mask = torch.empty((L, n), dtype=torch.int32, device=dev)
+            body = torch.empty((L, 2 * 246), dtype=torch.int32, device=dev)
+            if traced:
+                server.trace(1)
+            for _ in range(2):
+                server.respond_batch_device([table] * L, [seat] * L, seeds.data_ptr(), bodies.data_ptr(),
+                                            mask.data_ptr(), body.data_ptr())
+            if traced:
+                t = server.kernel_times()
+                assert len(t) == 1 and t[0] > 0
+            torch.cuda.synchronize()
+            outs.append((mask.cpu(), body.cpu()))
+        assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
+        with pytest.raises(gpu.PirError, match="at most 1048576 launches"):
+            server.trace((1 << 20) + 1)
+        table.close()
+        seat.close()
+
+
 @pytest.mark.parametrize("d,gamma", [(2048, 512), (256, 32), (64, 8)])
 def test_device_wire_bytes_equal_the_host_codec(gpu, d, gamma):
     """Response::to_bytes packed by the kernel equals the host codec over the word arrays of
