@@ -25,7 +25,7 @@
 // companions, a*w - floor(a*w'/2^32)*m; products by the per-query rows (the table,
 // the seat) are Montgomery products a*b*2^-32 with the rows stored as b = x*2^32, so
 // they need no companion words in HBM or registers. Both land in [0, 2m).
-// Residues mod q stay lazy in [0, 4q) (q < 2^30); residues mod p (p < 2^31) stay canonical,
+// Residues mod q stay lazy in [0, 4q) (6q < 2^32); residues mod p (p < 2^31) stay canonical,
 // except the forward transforms' outputs, in [0, 2p).
 //
 // Layout: 8 slots per thread. Radix-8 passes over bits 0-2 (the thread's own slots
@@ -461,7 +461,7 @@ __global__ __launch_bounds__((1 << LOGN) / 8, 5) void k_sm_respond(SmCtxDev c, S
         for (int k = 0; k < 8; k++) {
             const uint32_t t1 = mont(A[k], S[k], q, c.qninv);
             const uint32_t t2 = mont(V[k], WQ[k], q, c.qninv);
-            const uint32_t t3 = shoup(t2 + 4 * q - v[k], c.pinv_n, q);
+            const uint32_t t3 = shoup(t2 + 4 * q - v[k], c.pinv_n, q);  // (0, 6q): 6q < 2^32
             const int r = (int)((opaque_v(rot[k >> 1]) >> (16 * (k & 1))) & 0xffffu);
             bq[k] = red(PQ[r] + mont(V[k], KQ[k], q, c.qninv), 2 * q);
             V[k] = red(t1 + t3, 2 * q);

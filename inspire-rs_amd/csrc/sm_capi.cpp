@@ -128,9 +128,10 @@ int inspire_sm_server_create(const inspire_sm_params *params, inspire_sm_server 
         sm_err(err, err_cap, "packed width %llu is below the 8 the gfx950 kernels run", (unsigned long long)gamma);
         return INSPIRE_ERR_INVALID;
     }
-    if (q >= (1ULL << 30)) {
-        sm_err(err, err_cap, "ciphertext prime %llu is not below the 2^30 the gfx950 kernels run",
-               (unsigned long long)q);
+    // the widest lazy sum mod q, t2 + 4q - v in the recursion step, is below 6q
+    if (q > 0xffffffffULL / 6) {
+        sm_err(err, err_cap, "ciphertext prime %llu is above the %llu (2^32 / 6) the gfx950 kernels' lazy residues fit in a word",
+               (unsigned long long)q, 0xffffffffULL / 6);
         return INSPIRE_ERR_INVALID;
     }
     int logn = 0;

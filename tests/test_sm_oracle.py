@@ -291,3 +291,11 @@ def test_invalid_sets_are_refused():
     p.q, p.special_prime = p.special_prime, p.q
     with pytest.raises(g.PirError, match="not above"):
         p.validate()
+    # a set the reference accepts but the kernels' lazy sums mod q (below 6q) do not fit in a
+    # word: 715,829,249 is prime, 1 mod 4096 and below p; refused at server creation, before
+    # any device call
+    p = g.ring2048_cell256()
+    p.q = 715_829_249
+    p.validate()
+    with pytest.raises(g.PirError, match=r"above the 715827882 \(2\^32 / 6\)"):
+        g.SmServer(p)
