@@ -47,13 +47,77 @@ __host__ __device__ __forceinline__ void seed_key(const uint8_t *seed, uint32_t 
                  (uint32_t)seed[4 * i + 3] << 24;
 }
 
-// rand 0.8 gen_range(0..q) acceptance for one u64 draw v: returns true and the
-// value when the low product word is inside the zone (q << lz(q)) - 1.
-__host__ __device__ __forceinline__ bool uniform_q_draw(uint64_t v, uint64_t &out) {
+// Acceptance bound of rand 0.8's gen_range(0..q) on the low product word: (q << lz(q)) - 1.
+constexpr uint64_t kZoneQ = (kQ << 4) - 1;  // lz(q) = 4
+
+// gen_range(0..q) for one u64 draw v against an acceptance bound on the low word of v*q:
+// returns true and the value when the draw is accepted.
+__host__ __device__ __forceinline__ bool uniform_q_draw_in(uint64_t v, uint64_t zone, uint64_t &out) {
     uint64_t hi, lo = mul_wide(v, kQ, hi);
-    const uint64_t zone = (kQ << 4) - 1;  // lz(q) = 4
     out = hi;
     return lo <= zone;
+}
+// rand 0.8 gen_range(0..q) acceptance for one u64 draw v (the reference's zone).
+__host__ __device__ __forceinline__ bool uniform_q_draw(uint64_t v, uint64_t &out) {
+    return uniform_q_draw_in(v, kZoneQ, out);
+}
+
+// Serial gen_range(0..q) expansion with rejection (poly.rs:657-685 through rand's
+// sample_single_inclusive): u64 draws of stream 0 of ChaCha20(key) in order, two keystream
+// words each, low word first (rand_core's BlockRng: every draw takes two words and a block
+// holds 16, so no draw straddles a block), each accepted value handed to put(index, value)
+// until n are out. The kernels' parallel expansion takes one block per lane and runs this only
+// when some lane saw a rejected draw; zone is kZoneQ there (the host tests pass narrower
+// zones to force rejections). The block's words are read at unrolled constant indices, so
+// they stay in registers.
+template <class Put>
+__host__ __device__ inline void replay_gen_range_q(const uint32_t key[8], int n, uint64_t zone, Put &&put) {
+    uint32_t w[16];
+    int got = 0;
+    for (uint64_t blk = 0; got < n; blk++) {
+        chacha20_block(key, blk, 0, w);
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            uint64_t a;
+            if (got < n && uniform_q_draw_in((uint64_t)w[2 * u] | (uint64_t)w[2 * u + 1] << 32, zone, a))
+                put(got++, a);
+        }
+    }
+}
+
+// u % m for a u64 draw and odd m < 2^32, by Barrett with recip = floor((2^64 - 1) / m): the
+// estimate floor(u * recip / 2^64) is the quotient or one below it.
+__host__ __device__ __forceinline__ uint32_t reduce64(uint64_t u, uint32_t m, uint64_t recip) {
+    uint64_t hi;
+    (void)mul_wide(u, recip, hi);
+    const uint64_t r = u - hi * m;  // [0, 2m)
+    return (uint32_t)(r >= m ? r - m : r);
+}
+
+// Serial expand_uniform (special_modulus/mod.rs:223-237) over `limbs` consecutive moduli of
+// one ChaCha20 stream: `count` residues per modulus, a draw at or above zone[l] =
+// (2^64 - 1) / m * m rejected, each accepted draw reduced mod m[l] and handed to
+// put(l, index, value). Used where a lane of the parallel expansion saw a rejected draw.
+template <class Put>
+__host__ __device__ inline void replay_expand_uniform(const uint32_t key[8], uint64_t stream, int limbs, int count,
+                                                      const uint32_t *m, const uint64_t *zone,
+                                                      const uint64_t *recip, Put &&put) {
+    uint32_t w[16];
+    int l = 0, got = 0;
+    for (uint64_t blk = 0; l < limbs; blk++) {
+        chacha20_block(key, blk, stream, w);
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const uint64_t v = (uint64_t)w[2 * u] | (uint64_t)w[2 * u + 1] << 32;
+            if (l < limbs && v < zone[l]) {
+                put(l, got, reduce64(v, m[l], recip[l]));
+                if (++got == count) {
+                    l++;
+                    got = 0;
+                }
+            }
+        }
+    }
 }
 
 }  // namespace inspire

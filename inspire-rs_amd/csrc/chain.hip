@@ -182,20 +182,6 @@ __device__ __forceinline__ void inverse_to(uint64_t s[4], uint64_t *x, const Pai
     }
 }
 
-// digits of a canonical coefficient c (gadget_decompose, external_product.rs:65-99;
-// rounded form for a dropped lowest digit, inspiring2.rs:1184-1225)
-template <bool EXACT>
-__device__ __forceinline__ void digits(uint64_t c, uint64_t &d0, uint64_t &d1) {
-    if (EXACT) {
-        d0 = c & 0xfffff;
-        d1 = (c >> 20) & 0xfffff;
-    } else {
-        uint64_t r = (c + (1ULL << 19)) >> 20;
-        d0 = r & 0xfffff;
-        d1 = r >> 20;
-    }
-}
-
 // The top of the inverse transform, the digit split and the top of both forward
 // transforms for this thread's share, then the remaining forward passes down to
 // bits 3-2. On return C and D hold the digit transforms after all passes but the
@@ -212,8 +198,8 @@ __device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi
             uint64_t x = C[lds_ix(e)], y = C[lds_ix(e + N / 2)];
             gs(x, y, twi[1]);
             uint64_t a0, a1, b0, b1;
-            digits<EXACT>(canon4(x), a0, a1);
-            digits<EXACT>(canon4(y), b0, b1);
+            gadget_digits<EXACT>(canon4(x), a0, a1);
+            gadget_digits<EXACT>(canon4(y), b0, b1);
             ct(a0, b0, twf[1]);
             ct(a1, b1, twf[1]);
             C[lds_ix(e)] = a0;
@@ -239,7 +225,7 @@ __device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi
         for (int k = 0; k < 4; k++) v[k] = C[lds_ix(base + (k << b))];
         gs4<LOGN>(v, b, hi, twi);
 #pragma unroll
-        for (int k = 0; k < 4; k++) digits<EXACT>(canon4(v[k]), u[k], w[k]);
+        for (int k = 0; k < 4; k++) gadget_digits<EXACT>(canon4(v[k]), u[k], w[k]);
         ct4<LOGN>(u, b, hi, twf);
         ct4<LOGN>(w, b, hi, twf);
 #pragma unroll
@@ -383,20 +369,7 @@ void k_chain(CtxDev c, BatchDev bt) {
         if (rejected) {
             // a draw outside the acceptance zone (probability ~2^-46 each): replay the
             // stream serially with rejection, as the reference's RNG does
-            if (tid == 0) {
-                uint64_t blk = 0;
-                int word = 16, got = 0;
-                uint32_t w[16];
-                while (got < N) {
-                    uint32_t lo32, hi32;
-                    if (word >= 16) { chacha20_block(key, blk++, 0, w); word = 0; }
-                    lo32 = w[word++];
-                    if (word >= 16) { chacha20_block(key, blk++, 0, w); word = 0; }
-                    hi32 = w[word++];
-                    uint64_t a;
-                    if (uniform_q_draw((uint64_t)lo32 | (uint64_t)hi32 << 32, a)) A[aix(got++)] = a;
-                }
-            }
+            if (tid == 0) replay_gen_range_q(key, N, kZoneQ, [&](int k, uint64_t a) { A[aix(k)] = a; });
         }
     }
     __syncthreads();

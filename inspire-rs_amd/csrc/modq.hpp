@@ -154,6 +154,26 @@ __host__ __device__ __forceinline__ uint64_t shoup_lazy(uint64_t a, uint64_t w, 
     return a * w - est * kQ;
 }
 
+// Base-2^20 digits of a canonical coefficient c, two rows.
+// Exact gadget (gadget_decompose, rgsw/external_product.rs:65-99): the rows of digits 0 and 1,
+// d0 = c mod 2^20, d1 = floor(c / 2^20) mod 2^20; the third digit is not split out (the
+// packing chain derives its transform from the others).
+// Approximate gadget (gadget_decompose_rounded / rounded_digit_rows, inspiring2.rs:1167-1225,
+// one dropped digit): r = floor((c + 2^19) / 2^20), the round-half-up quotient, then
+// d0 = r mod 2^20 and d1 = floor(r / 2^20), the top row keeping the rounding carry (it
+// equals 2^20 for c >= 2^60 - 2^19). So c = 2^20 (d0 + 2^20 d1) + e with e in [-2^19, 2^19).
+template <bool EXACT>
+__host__ __device__ __forceinline__ void gadget_digits(uint64_t c, uint64_t &d0, uint64_t &d1) {
+    if (EXACT) {
+        d0 = c & 0xfffff;
+        d1 = (c >> 20) & 0xfffff;
+    } else {
+        const uint64_t r = (c + (1ULL << 19)) >> 20;
+        d0 = r & 0xfffff;
+        d1 = r >> 20;
+    }
+}
+
 __host__ __device__ __forceinline__ uint32_t bitrev(uint32_t x, int bits) {
 #ifdef __HIP_DEVICE_COMPILE__
     return __builtin_bitreverse32(x) >> (32 - bits);

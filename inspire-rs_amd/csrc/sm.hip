@@ -317,12 +317,6 @@ __device__ __forceinline__ void inverse_to_lds(uint32_t v[8], uint32_t *X, const
     __syncthreads();
 }
 
-// u64 draw -> residue mod m (m < 2^31) by Barrett with floor(2^64 / m)
-__device__ __forceinline__ uint32_t reduce64(uint64_t u, uint32_t m, uint64_t recip) {
-    uint64_t r = u - __umul64hi(u, recip) * m;  // [0, 2m)
-    return (uint32_t)(r >= m ? r - m : r);
-}
-
 __device__ __forceinline__ int bitrev_n(int x, int logn) { return (int)(__brev((uint32_t)x) >> (32 - logn)); }
 
 // Slot of f that slot j of tau_t(f) reads: slot j evaluates at psi^(2 brv(j) + 1).
@@ -364,17 +358,9 @@ __global__ __launch_bounds__((1 << LOGN) / 8, 5) void k_sm_respond(SmCtxDev c, S
         if (rejected) {
             // a draw in the short last interval (probability < 2^-34 each): replay the
             // stream serially with rejection, as the reference's loop does
-            if (t == 0) {
-                uint64_t blk = 0;
-                int word = 16, got = 0;
-                while (got < N) {
-                    if (word >= 16) { chacha20_block(key, blk++, 0, w); word = 0; }
-                    const uint32_t lo = w[word++];
-                    if (word >= 16) { chacha20_block(key, blk++, 0, w); word = 0; }
-                    const uint64_t u = (uint64_t)lo | (uint64_t)w[word++] << 32;
-                    if (u < c.zone_q) X[swz(got++)] = reduce64(u, q, c.q_recip);
-                }
-            }
+            if (t == 0)
+                replay_expand_uniform(key, 0, 1, N, &q, &c.zone_q, &c.q_recip,
+                                      [&](int, int k, uint32_t v) { X[swz(k)] = v; });
             __syncthreads();
         }
     }
@@ -495,7 +481,7 @@ __global__ __launch_bounds__((1 << LOGN) / 8, 5) void k_sm_respond(SmCtxDev c, S
 #pragma unroll
             for (int b = 0; b < 8; b++) {
                 const int32_t nib = (int32_t)(word << (28 - 4 * b)) >> 28;
-                s[b] += (int64_t)(cv * nib);
+                s[b] += (int64_t)cv * nib;  // |cv| <= q/2 < 2^30: the product needs 64 bits
             }
         }
         int64_t *part = reinterpret_cast<int64_t *>(PQ);  // R x 8W = 8T i64 over PQ and PP
@@ -671,21 +657,12 @@ __global__ __launch_bounds__((1 << LOGN) / 8) void k_sm_seat(SmCtxDev c, const u
     }
     __syncthreads();
     if (rejected) {
+        // a draw in the short last interval of either limb: replay both limbs serially
         if (t == 0) {
-            uint64_t blk = 0;
-            int word = 16, got = 0;
-            while (got < 2 * N) {
-                if (word >= 16) { chacha20_block(key, blk++, kStream, w); word = 0; }
-                const uint32_t lo = w[word++];
-                if (word >= 16) { chacha20_block(key, blk++, kStream, w); word = 0; }
-                const uint64_t u = (uint64_t)lo | (uint64_t)w[word++] << 32;
-                if (got < N) {
-                    if (u < c.zone_q) { XQ[swz(got)] = reduce64(u, q, c.q_recip); got++; }
-                } else if (u < c.zone_p) {
-                    XP[swz(got - N)] = reduce64(u, p, c.p_recip);
-                    got++;
-                }
-            }
+            const uint32_t m[2] = {q, p};
+            const uint64_t zone[2] = {c.zone_q, c.zone_p}, recip[2] = {c.q_recip, c.p_recip};
+            replay_expand_uniform(key, kStream, 2, N, m, zone, recip,
+                                  [&](int l, int k, uint32_t v) { (l ? XP : XQ)[swz(k)] = v; });
         }
         __syncthreads();
     }

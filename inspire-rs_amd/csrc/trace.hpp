@@ -39,12 +39,13 @@ struct KernelTrace {
     // the event slots of the next launch, or null once the armed launches are recorded
     hipEvent_t *next() { return n < cap ? &ev[n * marks] : nullptr; }
     void done() { n++; }
-    // durations of kernel k of each recorded launch (ms), after the last one completes;
-    // disarms
+    // durations of kernel k of each recorded launch (ms), once every recorded launch has
+    // completed (the launches may sit on different streams, so each one's last event is
+    // waited for, not only the last launch's); disarms
     hipError_t read(float *const *out, int kernels, size_t out_cap, size_t *count) {
         const size_t m = n < out_cap ? n : out_cap;
         hipError_t r = hipSuccess;
-        if (n) r = hipEventSynchronize(ev[(n - 1) * marks + marks - 1]);
+        for (size_t i = 0; i < n && r == hipSuccess; i++) r = hipEventSynchronize(ev[i * marks + marks - 1]);
         for (size_t i = 0; i < m && r == hipSuccess; i++)
             for (int k = 0; k < kernels && r == hipSuccess; k++)
                 r = hipEventElapsedTime(&out[k][i], ev[i * marks + k], ev[i * marks + k + 1]);

@@ -123,17 +123,22 @@ void orc_chacha_fill_bytes(orc_chacha *r, uint8_t *out, size_t len) {
 
 static int clz64(uint64_t x) { return x ? __builtin_clzll(x) : 64; }
 
-uint64_t orc_gen_range_u64(orc_chacha *r, uint64_t low, uint64_t high_excl) {
-    uint64_t high = high_excl - 1;
-    uint64_t range = high - low + 1;
-    if (range == 0) return orc_chacha_next_u64(r);
-    uint64_t zone = (range << clz64(range)) - 1;
+/* sample_single_inclusive's loop for a nonzero range and an acceptance bound `zone` on the
+ * low word of the widening product */
+static uint64_t gen_range_zone(orc_chacha *r, uint64_t low, uint64_t range, uint64_t zone) {
     for (;;) {
         uint64_t v = orc_chacha_next_u64(r);
         unsigned __int128 m = (unsigned __int128)v * range;
         uint64_t hi = (uint64_t)(m >> 64), lo = (uint64_t)m;
         if (lo <= zone) return low + hi;
     }
+}
+
+uint64_t orc_gen_range_u64(orc_chacha *r, uint64_t low, uint64_t high_excl) {
+    uint64_t high = high_excl - 1;
+    uint64_t range = high - low + 1;
+    if (range == 0) return orc_chacha_next_u64(r);
+    return gen_range_zone(r, low, range, (range << clz64(range)) - 1);
 }
 
 int64_t orc_gen_range_i64_incl(orc_chacha *r, int64_t low, int64_t high) {
@@ -159,4 +164,12 @@ void orc_expand_seed(const uint8_t seed[32], size_t n, uint64_t q, uint64_t *a) 
     orc_chacha r;
     orc_chacha_from_seed(&r, seed);
     for (size_t i = 0; i < n; i++) a[i] = orc_gen_range_u64(&r, 0, q);
+}
+
+/* Test hook: orc_expand_seed with the acceptance bound given instead of (q << lz(q)) - 1, so
+ * the tests can force rejections and check the kernels' serial replay against this loop. */
+void orc_expand_seed_zone(const uint8_t seed[32], size_t n, uint64_t q, uint64_t zone, uint64_t *a) {
+    orc_chacha r;
+    orc_chacha_from_seed(&r, seed);
+    for (size_t i = 0; i < n; i++) a[i] = gen_range_zone(&r, 0, q, zone);
 }

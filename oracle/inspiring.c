@@ -231,6 +231,9 @@ static void decompose(const orc_pack *p, const uint64_t *c, uint64_t *digits /* 
     }
 }
 
+/* Test hook: the digit split above over n coefficients (key_rows x n digits out). */
+void orc_decompose_digits(const orc_pack *p, const uint64_t *c, uint64_t *digits) { decompose(p, c, digits); }
+
 /* fold_rotations_backward fused with packing_online_gather: r_all (gamma x n, NTT) is
  * consumed; a_hat and sum_b (coefficient form) are produced. */
 static void recursion_and_online(const orc_pack *p, const uint64_t *w_mask_ntt, const uint64_t *y_ntt,

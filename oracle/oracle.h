@@ -134,6 +134,8 @@ int orc_automorph_table(const orc_pack *p, size_t t, uint32_t *out);
 /* w_mask rows as the offline keys hold them (kept_mask_rows + expand_mask, inspiring2.rs:839-887),
  * coefficient form, key_rows x n */
 void orc_expand_w_mask(const orc_pack *p, const uint8_t w_seed[32], uint64_t *w_mask);
+/* the recursion's gadget digit split of n canonical coefficients, key_rows x n (test hook) */
+void orc_decompose_digits(const orc_pack *p, const uint64_t *c, uint64_t *digits);
 
 /* rotation_sums (inspiring2.rs:964-1004): inputs_ntt count x n, out gamma x n (NTT form) */
 void orc_rotation_sums(const orc_pack *p, const uint64_t *inputs_ntt, size_t count, uint64_t *out);
@@ -166,6 +168,7 @@ size_t orc_encode_response(size_t n, uint64_t modulus, const uint64_t *a_hat, co
 
 /* query a from its seed (Poly::from_seed_moduli, poly.rs:682 -> :657) */
 void orc_expand_seed(const uint8_t seed[32], size_t n, uint64_t q, uint64_t *a);
+void orc_expand_seed_zone(const uint8_t seed[32], size_t n, uint64_t q, uint64_t zone, uint64_t *a);
 
 /* ---------------- client side of the packing-key KAT cells (client.c) ---------------- */
 typedef struct orc_cell orc_cell;
@@ -227,6 +230,8 @@ int sm_automorphism_tables(const sm_ntt *c, const size_t *exps, size_t count, ui
 int sm_ring_init(sm_ring *r, size_t d, size_t gamma, uint64_t q, uint64_t p, int with_tables);
 void sm_ring_free(sm_ring *r);
 void sm_expand_query_mask(const sm_ring *r, const uint8_t seed[32], uint32_t *a);
+void sm_expand_uniform_zone(const uint8_t seed[32], uint64_t stream, int limbs, size_t count, const uint32_t *m,
+                            const uint64_t *zone, uint32_t *out);
 void sm_key_mask(const sm_ring *r, const uint8_t seed[32], uint32_t *w_q, uint32_t *w_p);
 uint32_t sm_p_to_q_centred(const sm_ring *r, uint32_t x);
 uint32_t sm_q_to_p_centred(const sm_ring *r, uint32_t c);

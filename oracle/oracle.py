@@ -53,6 +53,7 @@ _lib.orc_store_build.argtypes = [ctypes.POINTER(_Pack), _u64p, _u64p]
 _lib.orc_fold_b.argtypes = [ctypes.POINTER(_Pack), _u64p, _u64p, _u64p]
 _lib.orc_automorph_table.argtypes = [ctypes.POINTER(_Pack), ctypes.c_size_t, _u32p]
 _lib.orc_expand_w_mask.argtypes = [ctypes.POINTER(_Pack), _u8p, _u64p]
+_lib.orc_decompose_digits.argtypes = [ctypes.POINTER(_Pack), _u64p, _u64p]
 _lib.orc_mod_switch.argtypes = [_u64p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64, _u64p]
 _lib.orc_encode_response.argtypes = [ctypes.c_size_t, ctypes.c_uint64, _u64p, _u64p, ctypes.c_size_t, _u8p]
 _lib.orc_encode_response.restype = ctypes.c_size_t
@@ -163,6 +164,14 @@ class Pack:
         _lib.orc_expand_w_mask(ctypes.byref(self._p), s, _p64(out))
         return out
 
+    def decompose(self, coeffs):
+        """The recursion's digit split of canonical coefficients: key_rows x len(coeffs)."""
+        c = np.ascontiguousarray(coeffs, dtype=np.uint64)
+        assert c.size == self.n
+        out = np.zeros((self.rows, self.n), dtype=np.uint64)
+        _lib.orc_decompose_digits(ctypes.byref(self._p), _p64(c), _p64(out))
+        return out
+
     def store_build(self, columns):
         columns = np.ascontiguousarray(columns, dtype=np.uint64)
         out = np.zeros((self.gamma, self.n), dtype=np.uint64)
@@ -218,6 +227,18 @@ def expand_seed(seed, n, q=Q):
     return out
 
 
+_lib.orc_expand_seed_zone.argtypes = [_u8p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64, _u64p]
+
+
+def expand_seed_zone(seed, n, zone, q=Q):
+    """expand_seed with an explicit acceptance bound on the low product word (test hook: a
+    narrow bound forces the rejections the reference's bound makes rare)."""
+    out = np.zeros(n, dtype=np.uint64)
+    s, _keep = _seed(seed)
+    _lib.orc_expand_seed_zone(s, n, q, zone, _p64(out))
+    return out
+
+
 def gauss_stream_hash(sigma, seed, count):
     return _lib.orc_gauss_stream_hash(sigma, seed, count)
 
@@ -258,6 +279,20 @@ _lib.sm_ring_init.argtypes = [ctypes.POINTER(_SmRing), ctypes.c_size_t, ctypes.c
 _lib.sm_ring_free.argtypes = [ctypes.POINTER(_SmRing)]
 _lib.sm_key_mask.argtypes = [ctypes.POINTER(_SmRing), _u8p, _u32p, _u32p]
 _lib.sm_expand_query_mask.argtypes = [ctypes.POINTER(_SmRing), _u8p, _u32p]
+_lib.sm_expand_uniform_zone.argtypes = [_u8p, ctypes.c_uint64, ctypes.c_int, ctypes.c_size_t, _u32p, _u64p, _u32p]
+
+
+def sm_expand_uniform_zone(seed, stream, moduli, zones, count):
+    """expand_uniform over consecutive moduli of one ChaCha20 stream, each draw at or above its
+    zone rejected (test hook: narrow zones force the rejections). Returns limbs x count."""
+    m = np.ascontiguousarray(moduli, dtype=np.uint32)
+    z = np.ascontiguousarray(zones, dtype=np.uint64)
+    out = np.zeros((len(m), count), dtype=np.uint32)
+    s, _keep = _seed(seed)
+    _lib.sm_expand_uniform_zone(s, stream, len(m), count, m.ctypes.data_as(_u32p), _p64(z), out.ctypes.data_as(_u32p))
+    return out
+
+
 _lib.sm_p_to_q_centred.argtypes = [ctypes.POINTER(_SmRing), ctypes.c_uint32]
 _lib.sm_p_to_q_centred.restype = ctypes.c_uint32
 _lib.sm_q_to_p_centred.argtypes = [ctypes.POINTER(_SmRing), ctypes.c_uint32]

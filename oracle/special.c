@@ -201,14 +201,29 @@ void sm_ring_free(sm_ring *r) {
     r->tables = NULL;
 }
 
-/* expand_uniform (mod.rs:223-237): reject the short last interval of u64. */
-static void sm_expand_uniform(orc_chacha *rng, uint64_t m, uint32_t *out, size_t count) {
-    uint64_t zone = (UINT64_MAX / m) * m;
+/* expand_uniform (mod.rs:223-237) with the rejection bound as an argument */
+static void sm_expand_uniform_in(orc_chacha *rng, uint64_t m, uint64_t zone, uint32_t *out, size_t count) {
     for (size_t i = 0; i < count; i++) {
         uint64_t u;
         do u = orc_chacha_next_u64(rng); while (u >= zone);
         out[i] = (uint32_t)(u % m);
     }
+}
+
+/* expand_uniform (mod.rs:223-237): reject the short last interval of u64. */
+static void sm_expand_uniform(orc_chacha *rng, uint64_t m, uint32_t *out, size_t count) {
+    sm_expand_uniform_in(rng, m, (UINT64_MAX / m) * m, out, count);
+}
+
+/* Test hook: expand_uniform over `limbs` consecutive moduli of one stream (as Ring::key_mask
+ * draws its q limb then its p limb), each with the rejection bound given, so the tests can
+ * force rejections and check the kernels' serial replay against this loop. */
+void sm_expand_uniform_zone(const uint8_t seed[32], uint64_t stream, int limbs, size_t count, const uint32_t *m,
+                            const uint64_t *zone, uint32_t *out) {
+    orc_chacha rng;
+    orc_chacha_from_seed(&rng, seed);
+    if (stream) orc_chacha_set_stream(&rng, stream);
+    for (int l = 0; l < limbs; l++) sm_expand_uniform_in(&rng, m[l], zone[l], out + (size_t)l * count, count);
 }
 
 /* A query's mask a from its seed, coefficient form (server.rs:408). */
