@@ -143,6 +143,22 @@ int inspire_respond_batch_device_timed(inspire_ctx *ctx, const inspire_seat *sea
                                        size_t count, uint64_t switch_modulus, uint64_t *d_a_out,
                                        uint64_t *d_b_prefix_out, void *stream, float kernel_ms[2], char *err,
                                        size_t err_cap);
+/* inspire_respond_batch_device without a workspace, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference. It calls no hipMalloc, hipFree or allocating library
+ * routine: the DB scan's partial sums live in a library-owned device array of
+ * INSPIRE_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends), and the per-query
+ * status (as inspire_workspace_status reports it) goes to the caller's d_status (count int32).
+ * A batch needing more scratch (count * 16 * gamma * 32 bytes at ring 2048: 256 queries at
+ * gamma 256) is refused. Every call on one GPU shares that GPU's array: calls that overlap on
+ * one GPU (other threads, other streams, other contexts) race and must be serialised by the
+ * caller. A shard table that grew since the last launch is refused until one allocating call
+ * (any respond or store build) has refreshed it. */
+#define INSPIRE_NOALLOC_SCRATCH_BYTES (32u << 20)
+int inspire_respond_batch_device_noalloc(inspire_ctx *ctx, const inspire_seat *seat, const uint32_t *d_shard_ids,
+                                         const uint8_t *d_seeds, const uint64_t *d_b, size_t count,
+                                         uint64_t switch_modulus, uint64_t *d_a_out, uint64_t *d_b_prefix_out,
+                                         int32_t *d_status, void *stream, char *err, size_t err_cap);
 /* Kernel trace (measurement; no counterpart in the reference): arms ws for its next
  * `launches` device-buffer launches (at most 2^20; 0 disarms), each of which then records HIP
  * events on its own stream around the DB-scan and recursion kernels without waiting, so a

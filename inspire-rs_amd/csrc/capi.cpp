@@ -48,6 +48,36 @@ int build_store(inspire_ctx *ctx, inspire_shard *sh, char *err, size_t err_cap) 
 
 namespace {
 
+// --- scratch of the allocation-free entry point (inspire_respond_batch_device_noalloc)
+// The DB scan's partial sums (and the parked b_poly) of one launch live in this file-scope
+// device array instead of a workspace: every GPU the library runs on holds its own copy, of
+// INSPIRE_NOALLOC_SCRATCH_BYTES, until the process ends. In host code `scratch` names a
+// host-side stand-in, not device memory: kernels and copies get the per-device address that
+// hipGetSymbolAddress returns, looked up once per device when a context is created there.
+static __device__ __align__(256) uint64_t scratch[INSPIRE_NOALLOC_SCRATCH_BYTES / 8];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
+
+// Caller has `device` current.
+hipError_t scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return hipSuccess;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device] = ptr;
+    return hipSuccess;
+}
+
+void *scratch_of(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch[device];
+}
+
 int register_shard(inspire_ctx *ctx, uint32_t id, const std::vector<uint16_t> &rec, inspire_shard **out, char *err,
                    size_t err_cap) {
     if (id < ctx->owners.size() && ctx->owners[id]) {
@@ -172,7 +202,9 @@ int inspire_ctx_create(const inspire_params *params, const uint8_t w_seed[32], u
         inv_p[m] = pair(inv[m]);
     }
 
-    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    hipError_t e = hipGetDevice(&ctx->device);
+    if (e == hipSuccess) e = scratch_setup(ctx->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     DevBuf d_seed;
     if (e == hipSuccess) e = ctx->psi.alloc(n * 8);
     if (e == hipSuccess) e = ctx->psi_inv.alloc(n * 8);
@@ -488,6 +520,69 @@ int inspire_respond_batch_device(inspire_ctx *ctx, const inspire_seat *seat, ins
                                  void *stream, char *err, size_t err_cap) {
     return respond_device(ctx, seat, ws, d_shard_ids, d_seeds, d_b, count, switch_modulus, d_a_out, d_b_prefix_out,
                           stream, nullptr, err, err_cap);
+}
+
+int inspire_respond_batch_device_noalloc(inspire_ctx *ctx, const inspire_seat *seat, const uint32_t *d_shard_ids,
+                                         const uint8_t *d_seeds, const uint64_t *d_b, size_t count,
+                                         uint64_t switch_modulus, uint64_t *d_a_out, uint64_t *d_b_prefix_out,
+                                         int32_t *d_status, void *stream, char *err, size_t err_cap) {
+    if (!ctx || (count && (!d_shard_ids || !d_seeds || !d_b || !d_a_out || !d_b_prefix_out || !d_status))) {
+        set_err(err, err_cap, "null argument");
+        return INSPIRE_ERR_INVALID;
+    }
+    int rc = check_batch(ctx, seat, nullptr, count, err, err_cap);
+    if (rc) return rc;
+    if (switch_modulus >= kSwitchModulusLimit) {
+        set_err(err, err_cap, "switch modulus %llu is not below 2^46 (the 36- and 45-bit targets are implemented)",
+                (unsigned long long)switch_modulus);
+        return INSPIRE_ERR_INVALID;
+    }
+    const size_t per_query = (size_t)ctx->ks() * ctx->gamma * 4 * sizeof(uint64_t);
+    if (count > INSPIRE_NOALLOC_SCRATCH_BYTES / per_query) {
+        set_err(err, err_cap,
+                "batch of %zu queries needs %zu bytes of scan scratch; the allocation-free entry point holds %u "
+                "bytes per GPU, at most %zu queries at this ring and width",
+                count, count * per_query, (unsigned)INSPIRE_NOALLOC_SCRATCH_BYTES,
+                (size_t)INSPIRE_NOALLOC_SCRATCH_BYTES / per_query);
+        return INSPIRE_ERR_INVALID;
+    }
+    if (count == 0) return INSPIRE_OK;
+    void *scr = scratch_of(ctx->device);
+    if (!scr) {
+        set_err(err, err_cap, "no allocation-free scratch on device %d", ctx->device);
+        return INSPIRE_ERR_DEVICE;
+    }
+    CtxDev dev;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        // the device shard table is refreshed in place when it fits (a copy, no allocation); a
+        // table that has outgrown its buffer needs one allocating call first
+        if (ctx->table_dirty && (!ctx->shard_table.p || ctx->shard_table.bytes < ctx->shards.size() * sizeof(ShardDev))) {
+            set_err(err, err_cap,
+                    "the shard table outgrew its device buffer since the last launch; make one allocating call "
+                    "(a respond or inspire_shard_precompute_packing) before the allocation-free one");
+            return INSPIRE_ERR_INVALID;
+        }
+        rc = sync_table(ctx, err, err_cap);
+        if (rc) return rc;
+        dev = ctx->dev();
+    }
+    BatchDev b;
+    b.count = (int)count;
+    b.shard_ids = d_shard_ids;
+    b.seeds = d_seeds;
+    b.yprimes = nullptr;
+    b.a_in = nullptr;
+    b.b = d_b;
+    b.yprime = seat->yprime.as<Pair>();
+    b.scan_part = static_cast<uint64_t *>(scr);
+    b.ks = ctx->ks();
+    b.a_out = d_a_out;
+    b.b_out = d_b_prefix_out;
+    b.status = d_status;
+    b.switch_modulus = switch_modulus;
+    HIP_TRY(launch_respond(dev, b, static_cast<hipStream_t>(stream)));
+    return INSPIRE_OK;
 }
 
 int inspire_workspace_trace(inspire_workspace *ws, size_t launches, char *err, size_t err_cap) {

@@ -133,6 +133,7 @@ struct inspire_ctx {
     std::vector<inspire_shard *> owners;
     bool table_dirty = true;
     int queries_per_cu = 1;
+    int device = 0;  // the GPU the context's memory lives on (current at creation)
     inspire_workspace *default_ws = nullptr;
     std::vector<void *> retired;  // superseded shard tables
     // the host-buffer entry points and store builds launch here and wait for this stream

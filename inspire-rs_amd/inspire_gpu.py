@@ -85,6 +85,7 @@ def lib():
         L.inspire_workspace_create.argtypes = [vp, sz, pp, cp, sz]
         L.inspire_workspace_destroy.argtypes = [vp]
         L.inspire_respond_batch_device.argtypes = [vp, vp, vp, vp, vp, vp, sz, u64, vp, vp, vp, cp, sz]
+        L.inspire_respond_batch_device_noalloc.argtypes = [vp, vp, vp, vp, vp, sz, u64, vp, vp, vp, vp, cp, sz]
         L.inspire_respond_batch_device_timed.argtypes = [vp, vp, vp, vp, vp, vp, sz, u64, vp, vp, vp, vp, cp, sz]
         L.inspire_workspace_trace.argtypes = [vp, sz, cp, sz]
         L.inspire_workspace_kernel_times.argtypes = [vp, vp, vp, sz, vp, cp, sz]
@@ -341,6 +342,18 @@ class Context:
         _call(lib().inspire_respond_batch_device, self._h, seat._h, ws._h, ctypes.c_void_p(d_ids),
               ctypes.c_void_p(d_seeds), ctypes.c_void_p(d_b), count, switch_modulus, ctypes.c_void_p(d_a_out),
               ctypes.c_void_p(d_b_out), ctypes.c_void_p(stream))
+
+
+def respond_batch_device_noalloc(ctx, seat, d_ids, d_seeds, d_b, count, d_a_out, d_b_out, d_status, stream=0,
+                                 switch_modulus=0):
+    """inspire_respond_batch_device with no workspace: the library's per-GPU scratch array
+    holds the scan partials and d_status receives the per-query status (count int32)."""
+    _call(lib().inspire_respond_batch_device_noalloc, ctx._h, seat._h, ctypes.c_void_p(d_ids),
+          ctypes.c_void_p(d_seeds), ctypes.c_void_p(d_b), count, switch_modulus, ctypes.c_void_p(d_a_out),
+          ctypes.c_void_p(d_b_out), ctypes.c_void_p(d_status), ctypes.c_void_p(stream))
+
+
+NOALLOC_SCRATCH_BYTES = 32 << 20
 
 
 class RespondGraph:
