@@ -83,8 +83,12 @@ int inspire_shard_upload_records(inspire_ctx *ctx, uint32_t shard_id, const uint
 int inspire_shard_precompute_packing(inspire_ctx *ctx, inspire_shard *shard, char *err, size_t err_cap);
 int inspire_shard_has_store(const inspire_shard *shard);
 /* Overwrite one column value in place without rebuilding the store (a shard edited
- * after its store was built; tests/shard_packing_store.rs:328-368). The next respond
- * against this shard is refused with INSPIRE_ERR_STORE. */
+ * after its store was built; tests/shard_packing_store.rs:328-368). After
+ * inspire_shard_precompute_packing, the next respond against this shard is refused with
+ * INSPIRE_ERR_STORE. A shard that never had precompute_packing (its store, if any, was built by
+ * a host-buffer respond on the way) is packed from its edited columns on the next respond, as
+ * the reference packs a store-less shard; device-buffer launches see no store for it (status -1)
+ * until then. */
 int inspire_shard_write_value(inspire_shard *shard, uint32_t record, uint32_t column, uint16_t value, char *err,
                               size_t err_cap);
 /* HBM bytes held by the shard: records + store + tag key + tags. */
@@ -135,6 +139,10 @@ int inspire_respond_batch_device(inspire_ctx *ctx, const inspire_seat *seat, ins
                                  const uint32_t *d_shard_ids, const uint8_t *d_seeds, const uint64_t *d_b,
                                  size_t count, uint64_t switch_modulus, uint64_t *d_a_out,
                                  uint64_t *d_b_prefix_out, void *stream, char *err, size_t err_cap);
+/* Destroying a shard, seat or workspace waits for the device launches of its context enqueued
+ * before the call (their fence events), not for the whole device; a launch enqueued afterwards
+ * that names a destroyed shard gets status -1. inspire_respond_batch_device_noalloc records no
+ * fence event: nothing it reads may be destroyed while it runs. */
 /* inspire_respond_batch_device, then waits: kernel_ms[0] receives the DB-scan kernel's
  * duration and kernel_ms[1] the recursion kernel's, from HIP events recorded on
  * `stream` around each launch. For measurement. */

@@ -4,7 +4,7 @@
 
 namespace inspire_rt {
 
-int build_store(inspire_ctx *ctx, inspire_shard *sh, char *err, size_t err_cap) {
+int build_store(inspire_ctx *ctx, inspire_shard *sh, bool lazy, char *err, size_t err_cap) {
     const size_t n = ctx->n, g = ctx->gamma;
     // key words below 2^61 - 1, drawn as the reference draws its tag key
     // (shard_store.rs:127-135): 61 random bits, rejected at or above the prime
@@ -36,6 +36,7 @@ int build_store(inspire_ctx *ctx, inspire_shard *sh, char *err, size_t err_cap) 
                                ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     sh->has_store = true;
+    sh->lazy_store = lazy;
     ShardDev &d = ctx->shards[sh->id];
     d.store = sh->store.as<Pair>();
     d.tag_key = sh->tag_key.as<uint64_t>();
@@ -224,10 +225,10 @@ int inspire_ctx_create(const inspire_params *params, const uint8_t w_seed[32], u
     // kept_mask_rows (inspiring2.rs:839-855): stream 0 for the exact gadget, gamma otherwise
     if (e == hipSuccess)
         e = launch_expand_mask(ctx->logn, ctx->rows, d_seed.as<uint8_t>(), ctx->dropped ? num_columns : 0,
-                               ctx->dropped, ctx->psi.as<uint64_t>(), ctx->inv_n, ctx->wprime.as<Pair>(), 0);
+                               ctx->dropped, ctx->psi.as<uint64_t>(), ctx->inv_n, ctx->wprime.as<Pair>(), ctx->stream);
     if (e == hipSuccess && ctx->rows == 3)
-        e = launch_fold_rows(ctx->wprime.as<Pair>(), (int)n, ctx->kx.w, ctx->k0.w, ctx->k1.w, 0);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
+        e = launch_fold_rows(ctx->wprime.as<Pair>(), (int)n, ctx->kx.w, ctx->k0.w, ctx->k1.w, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         set_err(err, err_cap, "HIP failure creating the context: %s", hipGetErrorString(e));
         delete ctx;
@@ -315,7 +316,7 @@ int inspire_shard_upload_records(inspire_ctx *ctx, uint32_t shard_id, const uint
 int inspire_shard_precompute_packing(inspire_ctx *ctx, inspire_shard *shard, char *err, size_t err_cap) {
     if (!ctx || !shard || shard->ctx != ctx) { set_err(err, err_cap, "shard belongs to another context"); return INSPIRE_ERR_INVALID; }
     std::lock_guard<std::mutex> lk(ctx->mu);
-    return build_store(ctx, shard, err, err_cap);
+    return build_store(ctx, shard, false, err, err_cap);
 }
 
 int inspire_shard_has_store(const inspire_shard *shard) { return shard && shard->has_store ? 1 : 0; }
@@ -331,6 +332,18 @@ int inspire_shard_write_value(inspire_shard *shard, uint32_t record, uint32_t co
     }
     size_t off = ((size_t)record * ctx->gamma + column) * 2;
     HIP_TRY(hipMemcpy(static_cast<char *>(shard->records.p) + off, &value, 2, hipMemcpyHostToDevice));
+    if (shard->has_store && shard->lazy_store) {
+        // a store the shard never asked for: forget it, so the next respond packs the edited
+        // columns as the reference's store-less path does (its buffers are reused by the rebuild)
+        shard->has_store = false;
+        shard->lazy_store = false;
+        if (shard->id < ctx->shards.size() && ctx->owners[shard->id] == shard) {
+            ctx->shards[shard->id].store = nullptr;
+            ctx->table_dirty = true;
+            int rc = sync_table(ctx, err, err_cap);
+            if (rc) return rc;
+        }
+    }
     return INSPIRE_OK;
 }
 
@@ -342,15 +355,33 @@ size_t inspire_shard_resident_bytes(const inspire_shard *shard) {
 void inspire_shard_destroy(inspire_shard *shard) {
     if (!shard) return;
     inspire_ctx *ctx = shard->ctx;
+    // Two phases. First the host forgets the shard (host-buffer calls naming it are refused)
+    // and every launch enqueued so far finishes with the shard as it was: those launches were
+    // enqueued under the lock with a fence event behind them, and store builds and host-buffer
+    // responds wait for their own stream before they release it.
+    std::vector<hipEvent_t> pending;
+    bool owned = false;
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
-        if (shard->id < ctx->owners.size() && ctx->owners[shard->id] == shard) {
-            ctx->owners[shard->id] = nullptr;
+        owned = shard->id < ctx->owners.size() && ctx->owners[shard->id] == shard;
+        if (owned) ctx->owners[shard->id] = nullptr;
+        pending = ctx->fence.snapshot();
+    }
+    (void)wait_for(pending);
+    // Then the device table drops it, so launches from here on get status -1 for it; the ones
+    // enqueued between the two phases may have read it either way and are waited for too.
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        // (unless a new shard was uploaded at the id meanwhile)
+        if (owned && ctx->shards[shard->id].records == shard->records.as<uint16_t>()) {
             ctx->shards[shard->id] = ShardDev{nullptr, nullptr, nullptr, nullptr};
             ctx->table_dirty = true;
+            char e[160];
+            (void)sync_table(ctx, e, sizeof e);
         }
+        pending = ctx->fence.snapshot();
     }
-    (void)hipDeviceSynchronize();
+    (void)wait_for(pending);
     delete shard;
 }
 
@@ -390,10 +421,10 @@ int inspire_seat_create(inspire_ctx *ctx, const uint64_t *y_body, uint32_t rows,
     if (e == hipSuccess) e = hipMemcpy(tmp.p, y_body, rows * n * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = launch_seat(ctx->logn, (int)rows, tmp.as<uint64_t>(), ctx->psi.as<uint64_t>(), ctx->inv_n,
-                        seat->yprime.as<Pair>(), 0);
+                        seat->yprime.as<Pair>(), ctx->stream);
     if (e == hipSuccess && rows == 3)
-        e = launch_fold_rows(seat->yprime.as<Pair>(), (int)n, ctx->kx.w, ctx->k0.w, ctx->k1.w, 0);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
+        e = launch_fold_rows(seat->yprime.as<Pair>(), (int)n, ctx->kx.w, ctx->k0.w, ctx->k1.w, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         set_err(err, err_cap, "HIP failure seating the packing key: %s", hipGetErrorString(e));
         delete seat;
@@ -405,9 +436,15 @@ int inspire_seat_create(inspire_ctx *ctx, const uint64_t *y_body, uint32_t rows,
 
 size_t inspire_seat_resident_bytes(const inspire_seat *seat) { return seat ? seat->yprime.bytes : 0; }
 
+// The launches enqueued so far on ctx (those that may read an object about to be freed).
+static std::vector<hipEvent_t> launches_so_far(inspire_ctx *ctx) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return ctx->fence.snapshot();
+}
+
 void inspire_seat_destroy(inspire_seat *seat) {
     if (!seat) return;
-    (void)hipDeviceSynchronize();
+    (void)wait_for(launches_so_far(seat->ctx));
     delete seat;
 }
 
@@ -424,7 +461,7 @@ int inspire_workspace_create(inspire_ctx *ctx, size_t max_count, inspire_workspa
 
 void inspire_workspace_destroy(inspire_workspace *ws) {
     if (!ws) return;
-    (void)hipDeviceSynchronize();
+    (void)wait_for(launches_so_far(ws->ctx));
     delete ws;
 }
 
@@ -445,14 +482,6 @@ static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_wo
                 (unsigned long long)switch_modulus);
         return INSPIRE_ERR_INVALID;
     }
-    CtxDev dev;
-    {
-        std::lock_guard<std::mutex> lk(ctx->mu);
-        rc = sync_table(ctx, err, err_cap);
-        if (rc) return rc;
-        dev = ctx->dev();
-    }
-    if (queries_per_cu) dev.queries_per_cu = queries_per_cu;  // a trial shape (autotune)
     BatchDev b;
     b.count = (int)count;
     b.shard_ids = d_shard_ids;
@@ -483,6 +512,13 @@ static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_wo
         return INSPIRE_ERR_INVALID;
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
+    // the shard table is read and the kernels are enqueued under the context's lock, with a
+    // fence event behind them, so destroying a shard, seat or workspace waits for this launch
+    std::unique_lock<std::mutex> lk(ctx->mu);
+    rc = sync_table(ctx, err, err_cap);
+    if (rc) return rc;
+    CtxDev dev = ctx->dev();
+    if (queries_per_cu) dev.queries_per_cu = queries_per_cu;  // a trial shape (autotune)
     if (!timing) {
         if (hipEvent_t *tr = ws->trace.next()) {
             // a traced launch: events around both kernels, read later, no wait
@@ -493,9 +529,10 @@ static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_wo
             if (e == hipSuccess) e = hipEventRecord(tr[2], st);
             HIP_TRY(e);
             ws->trace.done();
-            return INSPIRE_OK;
+        } else {
+            HIP_TRY(launch_respond(dev, b, st));
         }
-        HIP_TRY(launch_respond(dev, b, st));
+        HIP_TRY(ctx->fence.note(st));
         return INSPIRE_OK;
     }
     // per-kernel durations from events on the launch stream
@@ -506,6 +543,8 @@ static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_wo
     if (e == hipSuccess) e = hipEventRecord(ev[1], st);
     if (e == hipSuccess) e = launch_chain_only(dev, b, st);
     if (e == hipSuccess) e = hipEventRecord(ev[2], st);
+    if (e == hipSuccess) e = ctx->fence.note(st);
+    lk.unlock();
     if (e == hipSuccess) e = hipEventSynchronize(ev[2]);
     if (e == hipSuccess) e = hipEventElapsedTime(&timing[0], ev[0], ev[1]);
     if (e == hipSuccess) e = hipEventElapsedTime(&timing[1], ev[1], ev[2]);
@@ -641,6 +680,7 @@ int inspire_respond_batch_wire(inspire_ctx *ctx, const inspire_seat *seat, const
 }
 
 struct inspire_graph {
+    inspire_ctx *ctx = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ~inspire_graph() {
@@ -689,6 +729,7 @@ int inspire_respond_graph_create(inspire_ctx *ctx, const inspire_seat *seat, ins
     b.switch_modulus = switch_modulus;
     inspire_graph *g = new (std::nothrow) inspire_graph();
     if (!g) { set_err(err, err_cap, "out of host memory"); return INSPIRE_ERR_DEVICE; }
+    g->ctx = ctx;
     // the two kernels of the pipeline, captured on a private stream (no memset or copy nodes)
     hipStream_t cs = nullptr;
     hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
@@ -711,7 +752,9 @@ int inspire_respond_graph_create(inspire_ctx *ctx, const inspire_seat *seat, ins
 
 int inspire_graph_launch(inspire_graph *g, void *stream, char *err, size_t err_cap) {
     if (!g) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    std::lock_guard<std::mutex> lk(g->ctx->mu);
     HIP_TRY(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
+    HIP_TRY(g->ctx->fence.note(static_cast<hipStream_t>(stream)));
     return INSPIRE_OK;
 }
 
@@ -819,7 +862,7 @@ static int respond_host(inspire_ctx *ctx, const inspire_seat *seat, const uint32
     for (size_t k = 0; k < count; k++) {
         inspire_shard *sh = ctx->owners[shard_ids[k]];
         if (!sh->has_store) {
-            rc = build_store(ctx, sh, err, err_cap);
+            rc = build_store(ctx, sh, true, err, err_cap);
             if (rc) return rc;
         }
     }

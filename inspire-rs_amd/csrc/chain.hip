@@ -308,12 +308,16 @@ void k_chain(CtxDev c, BatchDev bt) {
     const int qi = blockIdx.x, tid = threadIdx.x, gamma = c.gamma;
     // the ids of device-buffer batches are not checked on the host: a query naming no shard
     // with a store is refused here, before anything reads the shard (the whole workgroup)
-    const uint32_t sid = bt.shard_ids[qi];
-    if (sid >= (uint32_t)c.num_shards || !c.shards[sid].store) {
+    // One thread reads the entry and the workgroup takes its copy from LDS, so every wave
+    // decides alike even if the table changes under a running launch.
+    __shared__ ShardDev SH;
+    if (tid == 0) SH = shard_entry(c.shards, c.num_shards, bt.shard_ids[qi]);
+    __syncthreads();
+    const ShardDev sh{uniform(SH.records), uniform(SH.store), uniform(SH.tag_key), uniform(SH.tags)};
+    if (!shard_ready(sh)) {
         if (tid == 0) bt.status[qi] = -1;
         return;
     }
-    const ShardDev sh = c.shards[sid];
     const Pair *twf = c.twf, *twi = c.twi;
     const Pair *Y = bt.yprimes ? bt.yprimes[qi] : bt.yprime;
     // Twiddles of the passes at base bit BL and up (ladder entries below n >> BL: 16 KiB at

@@ -19,6 +19,23 @@ struct ShardDev {
     const uint64_t *tags;     // gamma column tags mod 2^61 - 1
 };
 
+// The shard entry a query names, read once: zero when the id is past the table, and usable only
+// when every pointer is set (a shard destroyed, or its store still being built, while a launch
+// naming it runs is seen with some or all of them null, never half-freed: destroying a shard
+// waits for the launches enqueued before it).
+__device__ inline ShardDev shard_entry(const ShardDev *table, int num_shards, uint32_t sid) {
+    ShardDev z{nullptr, nullptr, nullptr, nullptr};
+    return sid < (uint32_t)num_shards ? table[sid] : z;
+}
+__device__ inline bool shard_ready(const ShardDev &s) { return s.records && s.store && s.tag_key && s.tags; }
+// A pointer every lane holds alike, kept in scalar registers.
+template <class T>
+__device__ inline T *uniform(T *p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return reinterpret_cast<T *>((uint64_t)hi << 32 | lo);
+}
+
 // Per-CRS constants (the reference's PackParams + OfflinePackingKeys,
 // src/inspiring/inspiring2.rs:28-60, 450-470), all on the device.
 struct CtxDev {

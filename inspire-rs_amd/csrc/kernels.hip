@@ -167,9 +167,8 @@ __global__ __launch_bounds__(kThreads) void k_scan(CtxDev c, BatchDev bt) {
     if (UNIFORM_ROW) split = __builtin_amdgcn_readfirstlane(split);
     // ids of device-buffer batches are not checked on the host: a query naming no shard with
     // a store is skipped here and refused by k_chain (status -1)
-    const uint32_t sid = bt.shard_ids[qi];
-    if (sid >= (uint32_t)c.num_shards || !c.shards[sid].store) return;
-    const ShardDev sh = c.shards[sid];
+    const ShardDev sh = shard_entry(c.shards, c.num_shards, bt.shard_ids[qi]);
+    if (!shard_ready(sh)) return;
     const uint64_t *b = bt.b + (size_t)qi * n;
     const int rows = n / bt.ks, r0 = split * rows;  // a multiple of 8: n >= 256, ks <= n / 128
     const uint64_t *rec = reinterpret_cast<const uint64_t *>(sh.records) + g;
@@ -215,9 +214,8 @@ __global__ __launch_bounds__(kThreads) void k_scan_narrow(CtxDev c, BatchDev bt)
     const int n = c.n, gamma = c.gamma;
     // ids of device-buffer batches are not checked on the host: a query naming no shard with
     // a store is skipped here and refused by k_chain (status -1)
-    const uint32_t sid = bt.shard_ids[qi];
-    if (sid >= (uint32_t)c.num_shards || !c.shards[sid].store) return;
-    const ShardDev sh = c.shards[sid];
+    const ShardDev sh = shard_entry(c.shards, c.num_shards, bt.shard_ids[qi]);
+    if (!shard_ready(sh)) return;
     const uint64_t *b = bt.b + (size_t)qi * n;
     const int rows = n / bt.ks, r0 = split * rows;
     const int lanes = kThreads / gamma;

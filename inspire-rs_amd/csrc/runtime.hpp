@@ -14,6 +14,7 @@
 
 #include "../../include/inspire_gpu.h"
 #include "device.hpp"
+#include "fence.hpp"
 #include "kernels.hpp"
 #include "modq.hpp"
 #include "trace.hpp"
@@ -104,6 +105,7 @@ struct inspire_shard {
     uint32_t id = 0;
     DevBuf records, store, tag_key, tags;
     bool has_store = false;
+    bool lazy_store = false;  // built by a respond, not by precompute_packing
 };
 
 struct inspire_seat {
@@ -139,11 +141,15 @@ struct inspire_ctx {
     // the host-buffer entry points and store builds launch here and wait for this stream
     // alone, not for the whole device (other callers' streams keep running)
     hipStream_t stream = nullptr;
+    // every launch that reads the shard table, a shard, a seat or a workspace records an event
+    // here under mu, so destroying one of them waits for those launches only
+    LaunchFence fence;
     ~inspire_ctx() {
         if (stream) {
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
         }
+        (void)wait_for(fence.snapshot());
         for (void *p : retired) (void)hipFree(p);
     }
 
@@ -182,8 +188,11 @@ inline int sync_table(inspire_ctx *ctx, char *err, size_t err_cap) {
     return INSPIRE_OK;
 }
 
-// Builds a shard's packing store on the device (capi.cpp); caller holds ctx->mu.
-int build_store(inspire_ctx *ctx, inspire_shard *sh, char *err, size_t err_cap);
+// Builds a shard's packing store on the device (capi.cpp); caller holds ctx->mu. lazy marks a
+// store built by a respond for a shard that was uploaded without one: the reference packs such
+// a shard from its columns on every query, so an edit of the shard drops this store (the next
+// respond builds it again from the edited columns) instead of failing its tag check.
+int build_store(inspire_ctx *ctx, inspire_shard *sh, bool lazy, char *err, size_t err_cap);
 
 inline int ws_reserve(inspire_workspace *ws, size_t count, char *err, size_t err_cap) {
     inspire_ctx *ctx = ws->ctx;

@@ -138,13 +138,13 @@ int inspire_server::launch(Slot &s, char *err, size_t err_cap) {
         q += c;
     }
     s.count = q;
-    CtxDev dev;
-    {
-        std::lock_guard<std::mutex> lk(ctx->mu);
-        int rc = sync_table(ctx, err, err_cap);
-        if (rc) return rc;
-        dev = ctx->dev();  // the shard table and launch shape as of this launch
-    }
+    // the shard table and launch shape as of this launch, read under the context's lock, which is
+    // held until the kernels are enqueued with a fence event behind them (a shard or seat
+    // destroyed meanwhile waits for this launch)
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    int rc = sync_table(ctx, err, err_cap);
+    if (rc) return rc;
+    const CtxDev dev = ctx->dev();
     // The slot's previous launch has completed (its done_ev, after its output copies), so its
     // buffers are free; the copies may run while the other slot's kernels do.
     HIP_TRY(hipMemcpyAsync(s.d_ids.p, s.h_ids, q * 4, hipMemcpyHostToDevice, copy_in));
@@ -168,6 +168,7 @@ int inspire_server::launch(Slot &s, char *err, size_t err_cap) {
     bd.status = s.status.as<int32_t>();
     bd.switch_modulus = s.reqs.front().switch_modulus;
     HIP_TRY(launch_respond(dev, bd, stream));
+    HIP_TRY(ctx->fence.note(stream));
     HIP_TRY(hipEventRecord(s.kern_ev, stream));
     HIP_TRY(hipStreamWaitEvent(copy_out, s.kern_ev, 0));
     HIP_TRY(hipMemcpyAsync(s.h_a, s.d_a.p, q * n * 8, hipMemcpyDeviceToHost, copy_out));
@@ -407,7 +408,7 @@ int inspire_server_submit(inspire_server *sv, uint64_t handle, const uint32_t *s
                 return INSPIRE_ERR_INVALID;
             }
             if (!ctx->owners[id]->has_store) {
-                int rc = build_store(ctx, ctx->owners[id], err, err_cap);
+                int rc = build_store(ctx, ctx->owners[id], true, err, err_cap);
                 if (rc) return rc;
             }
         }

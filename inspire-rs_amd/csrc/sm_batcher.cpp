@@ -137,7 +137,12 @@ int inspire_sm_batcher::launch(Slot &s, char *err, size_t err_cap) {
     SM_TRY(hipStreamWaitEvent(stream, s.in_ev, 0));
     SmBatchDev bd{(int)q, s.d_jobs.as<SmJob>(), s.d_seeds.as<uint8_t>(), s.d_bodies.as<uint32_t>(),
                   s.d_mask.as<uint32_t>(), s.d_body.as<uint32_t>()};
-    SM_TRY(sm_launch_respond(srv->dev, bd, stream));
+    {
+        // a fence event behind the kernel, so destroying a seat or table of the server waits for it
+        std::lock_guard<std::mutex> lk(srv->mu);
+        SM_TRY(sm_launch_respond(srv->dev, bd, stream));
+        SM_TRY(srv->fence.note(stream));
+    }
     SM_TRY(hipEventRecord(s.kern_ev, stream));
     SM_TRY(hipStreamWaitEvent(copy_out, s.kern_ev, 0));
     SM_TRY(hipMemcpyAsync(s.h_mask, s.d_mask.p, q * n * 4, hipMemcpyDeviceToHost, copy_out));

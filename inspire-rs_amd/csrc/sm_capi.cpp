@@ -209,7 +209,7 @@ int inspire_sm_server_create(const inspire_sm_params *params, inspire_sm_server 
 
 void inspire_sm_server_destroy(inspire_sm_server *s) {
     if (!s) return;
-    (void)hipDeviceSynchronize();
+    (void)wait_for(s->launches_so_far());
     delete s;
 }
 
@@ -238,8 +238,8 @@ int inspire_sm_table_create(inspire_sm_server *s, const uint8_t *cells, size_t l
     if (e == hipSuccess) e = hipMemset(t->cells.p, 0, n * cell);
     if (e == hipSuccess && len) e = hipMemcpy(t->cells.p, cells, len, hipMemcpyHostToDevice);
     if (e == hipSuccess)
-        e = sm_launch_table(s->dev, t->cells.as<uint8_t>(), s->cols.as<int8_t>(), t->step1.as<uint32_t>(), 0);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
+        e = sm_launch_table(s->dev, t->cells.as<uint8_t>(), s->cols.as<int8_t>(), t->step1.as<uint32_t>(), s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) {
         sm_err(err, err_cap, "HIP failure building the step-1 table: %s", hipGetErrorString(e));
         delete t;
@@ -254,7 +254,7 @@ size_t inspire_sm_table_resident_bytes(const inspire_sm_table *t) { return t ? t
 
 void inspire_sm_table_destroy(inspire_sm_table *t) {
     if (!t) return;
-    (void)hipDeviceSynchronize();
+    (void)wait_for(t->srv->launches_so_far());
     delete t;
 }
 
@@ -279,8 +279,8 @@ int inspire_sm_seat_create(inspire_sm_server *s, const uint8_t mask_seed[32], co
     if (e == hipSuccess) e = hipMemcpy(kq, key_q, n * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(kp, key_p, n * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(seed, mask_seed, 32, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = sm_launch_seat(s->dev, seed, kq, kp, st->words.as<uint32_t>(), 0);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = sm_launch_seat(s->dev, seed, kq, kp, st->words.as<uint32_t>(), s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) {
         sm_err(err, err_cap, "HIP failure seating the packing key: %s", hipGetErrorString(e));
         delete st;
@@ -294,7 +294,7 @@ size_t inspire_sm_seat_resident_bytes(const inspire_sm_seat *st) { return st ? s
 
 void inspire_sm_seat_destroy(inspire_sm_seat *st) {
     if (!st) return;
-    (void)hipDeviceSynchronize();
+    (void)wait_for(st->srv->launches_so_far());
     delete st;
 }
 
@@ -380,12 +380,13 @@ int inspire_sm_respond(inspire_sm_server *s, const inspire_sm_table *table, cons
 
 // The device entry points' job list, uploaded only when the handles change, so a repeated
 // batch launches with nothing but the kernel on the stream. A launch of this server on any
-// stream may still read the old list, so a change waits for the device first. Caller holds s->mu.
+// stream may still read the old list, so a change waits for the server's launches enqueued so
+// far (its fence), not for the whole device. Caller holds s->mu.
 static int upload_device_jobs(inspire_sm_server *s, const std::vector<SmJob> &jobs, char *err, size_t err_cap) {
     if (jobs.size() == s->djobs_host.size() &&
         std::memcmp(jobs.data(), s->djobs_host.data(), jobs.size() * sizeof(SmJob)) == 0)
         return INSPIRE_OK;
-    SM_TRY(hipDeviceSynchronize());
+    SM_TRY(wait_for(s->fence.snapshot()));
     if (jobs.size() > s->djobs_cap) {
         SM_TRY(s->djobs.alloc(jobs.size() * sizeof(SmJob)));
         s->djobs_cap = jobs.size();
@@ -481,6 +482,7 @@ int inspire_sm_respond_batch_device_wire(inspire_sm_server *s, size_t count, con
     b.wire_out = d_wire_out;
     b.wire_stride = wire_stride;
     SM_TRY(launch_traced(s, b, st));
+    SM_TRY(s->fence.note(st));
     return INSPIRE_OK;
 }
 
@@ -507,6 +509,7 @@ int inspire_sm_respond_batch_device(inspire_sm_server *s, size_t count, const in
     SmBatchDev b{(int)count, s->djobs.as<SmJob>(), d_seeds, d_bodies, d_mask_out, d_body_out};
     if (!kernel_ms) {
         SM_TRY(launch_traced(s, b, st));
+        SM_TRY(s->fence.note(st));
         return INSPIRE_OK;
     }
     hipEvent_t ev[2];
@@ -515,6 +518,7 @@ int inspire_sm_respond_batch_device(inspire_sm_server *s, size_t count, const in
     hipError_t e = hipEventRecord(ev[0], st);
     if (e == hipSuccess) e = sm_launch_respond(s->dev, b, st);
     if (e == hipSuccess) e = hipEventRecord(ev[1], st);
+    if (e == hipSuccess) e = s->fence.note(st);
     if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
     if (e == hipSuccess) e = hipEventElapsedTime(kernel_ms, ev[0], ev[1]);
     (void)hipEventDestroy(ev[0]);

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/inspire_gpu.h"
+#include "fence.hpp"
 #include "sm.hpp"
 #include "trace.hpp"
 
@@ -64,11 +65,19 @@ struct inspire_sm_server {
     hipStream_t stream = nullptr;
     // kernel events of the next device-buffer launches, when armed
     inspire::KernelTrace trace;
+    // an event behind every launch that reads this server's tables, seats or job list, recorded
+    // under mu: freeing one of them waits for those launches only, not the whole device
+    inspire::LaunchFence fence;
     ~inspire_sm_server() {
         if (stream) {
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
         }
+        (void)inspire::wait_for(fence.snapshot());
+    }
+    std::vector<hipEvent_t> launches_so_far() {
+        std::lock_guard<std::mutex> lk(mu);
+        return fence.snapshot();
     }
 };
 
