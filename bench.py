@@ -1,15 +1,17 @@
 """Benchmark of the MI355X InsPIRe respond path (one GPU).
 
-Workload (default): the batched TwoPacking respond over a 2^18 x 512-byte database
-(128 MiB, 128 shards of 2048 records, gamma = 256 columns), d = 2048, q = 2^60-2^14+1,
-every shard's packing store precomputed on the device, one seated client key, and
-batches of 64 seeded queries at independent uniform record indices. A step is one
-batch answered end to end (DB scan + packing recursion + online gather, 36-bit
+Headline workload: the batched exact-gadget TwoPacking respond over a 2^20 x 512-byte database
+(512 MiB, 512 shards of 2048 records, gamma = 256 columns), d = 2048, q = 2^60-2^14+1, every
+shard's packing store precomputed on the device, one seated client key, batches of 64 seeded
+queries at independent uniform record indices, 8 batches merged into each launch. A step is one
+launch (512 queries answered end to end: DB scan + packing recursion + online gather, 36-bit
 modulus switch); inputs are resident in HBM when the timed region starts.
 
-Beside it the same run measures the plain path (one query per launch over the 2^16 x
-512-byte database) and the CPU oracle (a C restatement of the reference algorithm)
-on a bounded sample, and reports per-kernel roofline figures from HIP events.
+Beside it the same run measures the plain path (one query per launch) over the same database,
+the 2^18-record batched config (64 and 512 queries per launch), the 2^16-record plain config,
+the served configs (approximate gadget over 2^20 records; special modulus), and the CPU oracle
+(a C restatement of the reference algorithm) on bounded samples, and reports per-kernel
+roofline figures from HIP events.
 """
 import argparse
 import json
@@ -77,12 +79,40 @@ def _timed_queries(run_one, budget_s, threads):
     return sum(counts), dt
 
 
-def cpu_baseline(budget_s=10.0):
-    """The CPU oracle (C restatement of the reference respond) on a bounded sample: seeded
-    d2048 gamma=256 queries against a prebuilt store, on all of the box's host cores (one
-    query per thread, as a multi-threaded server answers independent requests) and on one;
-    with the reference's AVX-512 IFMA multiply-accumulates (its simd-packing-offline feature)
-    where the host has them, and without. value is the all-core rate of the faster form."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _pct(xs):
+    xs = np.sort(np.asarray(xs, dtype=np.float64))
+    return {k: round(float(np.percentile(xs, q)), 3) for k, q in (("p50", 50), ("p95", 95), ("p99", 99))}
+
+
+def cpu_baseline(budget_s=10.0, rounds=30):
+    """The CPU oracle (C restatement of the reference respond) on bounded samples.
+
+    value: seeded d2048 gamma=256 exact-gadget queries against a prebuilt store (the headline's
+    per-query work) on all of the box's host cores, one query per thread as a multi-threaded
+    server answers independent requests, with the reference's AVX-512 IFMA multiply-accumulates
+    (its simd-packing-offline feature) where the host has them; scalar and one-core rates beside.
+
+    latency: the reference benches' method (respond_latency_bench.rs:29-86): warm-up, then
+    `rounds` rounds, each timing one query of every config on one thread in an order that
+    rotates from round to round, p50/p95/p99 per config and the 1-minute load average at each
+    round:
+      C0  d2048 gamma 16, 2048 x 32 B: respond_seeded_inspiring uncached (PackParams and the
+          offline keys built per call, no store) then extract_inspiring;
+      C2  d2048 gamma 256 exact gadget against a prebuilt store (C1-C2 and the headline);
+      C3  d2048_approximate_gadget gamma 256 against a prebuilt store (the served config);
+      C4  special modulus ring2048_cell256 against one prebuilt table and seat.
+    The oracle answers a query on one thread (the reference's rayon sections, the column fold
+    and step 1, are a few percent of a query; its recursion is serial)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
@@ -114,12 +144,72 @@ def cpu_baseline(budget_s=10.0):
     n1, dt1, nT, dtT = res[best]
     sn1, sdt1, snT, sdtT = res["scalar"]
     form = "with AVX-512 IFMA multiply-accumulates" if has_simd else "scalar (host has no AVX-512 IFMA)"
+
+    # --- per-query latency of each config, rounds in rotating order
+    c0 = oracle.Cell(N, 32, 0x5EED_C0)
+    c0_queries = [c0.query(int(i)) for i in np.random.default_rng(5).integers(0, 3 * N, rounds + 2)]
+    approx = oracle.Pack(N, GAMMA, 3, 1)
+    approx.set_simd(has_simd)
+    y2 = rng.integers(0, oracle.Q, (2, N), dtype=np.uint64)
+    astore = approx.store_build(cols)
+    ring = oracle.SmRing(cell_bytes=SM_CELL, record_bytes=SM_RECORD)
+    table = ring.table(sm_cells(rng, SM_N))
+    sm_seat = ring.seat(rng.integers(0, oracle.SM_Q, SM_N, dtype=np.uint32),
+                        rng.integers(0, oracle.SM_P, SM_N, dtype=np.uint32), bytes(32))
+    sm_avx = oracle.sm_avx512_available()
+    fast = simd if has_simd else scalar
+
+    def c0_one(k):
+        shard, seed, b = c0_queries[k]
+        p = oracle.Pack(N, c0.gamma)  # uncached: PackParams + offline keys per call
+        a_hat, bo = p.respond(c0.w_seed, oracle.expand_seed(seed, N), b, c0.columns(shard), c0.y_body)
+        return oracle.extract_inspiring(c0.secret, a_hat, bo[:c0.gamma], 32)
+
+    def c2_one(k):
+        r = np.random.default_rng(700 + k)
+        fast.respond(w_seed, oracle.expand_seed(r.bytes(32), N), r.integers(0, oracle.Q, N, dtype=np.uint64), cols,
+                     y, store)
+
+    def c3_one(k):
+        r = np.random.default_rng(800 + k)
+        approx.respond(w_seed, oracle.expand_seed(r.bytes(32), N), r.integers(0, oracle.Q, N, dtype=np.uint64),
+                       cols, y2, astore)
+
+    def c4_one(k):
+        r = np.random.default_rng(900 + k)
+        ring.respond(table, sm_seat, r.bytes(32), r.integers(0, oracle.SM_Q, SM_N, dtype=np.uint32), avx512=sm_avx)
+
+    configs = [("C0", c0_one), ("C2", c2_one), ("C3", c3_one), ("C4", c4_one)]
+    for _, fn in configs:  # warm-up
+        fn(rounds + 1)
+    lat = {name: [] for name, _ in configs}
+    loads = []
+    for r in range(rounds):
+        order = configs[r % len(configs):] + configs[:r % len(configs)]
+        for name, fn in order:
+            t0 = time.perf_counter()
+            fn(r)
+            lat[name].append((time.perf_counter() - t0) * 1e3)
+        loads.append(round(os.getloadavg()[0], 2))
+    latency = {name: {"ms": _pct(v), "queries_per_s_one_core": round(1e3 / float(np.mean(v)), 2)}
+               for name, v in lat.items()}
+    latency["C0"]["workload"] = ("d2048 gamma 16, 2048 x 32 B: respond_seeded_inspiring uncached (PackParams and "
+                                 "offline keys built per call, per-query packing from the columns) + "
+                                 "extract_inspiring")
+    latency["C2"]["workload"] = "d2048 gamma 256 exact gadget, prebuilt store" + (" (IFMA)" if has_simd else "")
+    latency["C3"]["workload"] = "d2048_approximate_gadget gamma 256, prebuilt store" + (" (IFMA)" if has_simd else "")
+    latency["C4"]["workload"] = ("special modulus ring2048_cell256, prebuilt table and seat"
+                                 + (" (AVX-512 build)" if sm_avx else ""))
     return {"value": round(nT / dtT, 3), "unit": "queries/s", "cores": T, "kind": "port",
-            "single_core": round(n1 / dt1, 3), "simd": "avx512ifma" if has_simd else None,
+            "cpu_model": _cpu_model(), "single_core": round(n1 / dt1, 3),
+            "simd": "avx512ifma" if has_simd else None,
             "without_simd": {"value": round(snT / sdtT, 3), "single_core": round(sn1 / sdt1, 3)},
             "sample": f"seeded d2048 gamma=256 queries against a prebuilt store (C restatement), {form}: "
                       f"{nT} on {T} threads in {dtT:.1f} s and {n1} on 1 thread in {dt1:.1f} s; scalar "
-                      f"{snT} on {T} threads in {sdtT:.1f} s and {sn1} on 1 thread in {sdt1:.1f} s"}
+                      f"{snT} on {T} threads in {sdtT:.1f} s and {sn1} on 1 thread in {sdt1:.1f} s",
+            "latency": {"rounds": rounds, "order": "rotating per round over C0, C2, C3, C4", "threads": 1,
+                        "load_avg_1min": {"min": min(loads), "max": max(loads), "per_round": loads},
+                        **latency}}
 
 
 def served(g, seconds, rate_qps, seed=7, capacity_queries=0, queries_per_cu=2):
@@ -458,22 +548,165 @@ def sm_served(g, srv, tables, rng, rate_qps, seconds, keys=256, capacity_queries
                         f"(PCIe-inclusive), continuous batching <= {max_launch} queries per launch"}
 
 
+def batched_leg(g, torch, dev, stream, ctx, seat, shards, B, M, launches, warmup, rng, tune=True, graph=True,
+                single=True, trace=True):
+    """Launches of M batches of B seeded queries merged (L = B M queries per launch) over the
+    context's `shards` stored shards, back to back on one stream, inputs resident in HBM."""
+    L = B * M
+    ws = g.Workspace(ctx, L)
+    total = warmup + launches
+    ids, seeds, bs = make_batches(rng, total * M, B, shards, g.Q)
+    d_ids = torch.from_numpy(ids.astype(np.int32)).to(dev).reshape(total, L)
+    d_seeds = torch.from_numpy(seeds).to(dev).reshape(total, L, 32)
+    d_b = torch.from_numpy(bs.view(np.int64)).to(dev).reshape(total, L, N)
+    d_a_out = torch.empty((L, N), dtype=torch.int64, device=dev)
+    d_b_out = torch.empty((L, GAMMA), dtype=torch.int64, device=dev)
+    torch.cuda.synchronize()
+
+    def issue(i):
+        ctx.respond_batch_device(seat, ws, d_ids[i].data_ptr(), d_seeds[i].data_ptr(), d_b[i].data_ptr(), L,
+                                 d_a_out.data_ptr(), d_b_out.data_ptr(), stream=stream, switch_modulus=Q36)
+
+    out = {"queries_per_launch": L, "merge": M, "batch": B, "launches": launches, "shards": shards}
+    if tune:
+        # launch shape picked by timing the three shapes on synthetic queries at this launch size
+        out["autotune"] = g.autotune(ctx, seat, ws, L)
+        log(f"bench: autotune at {L} queries/launch over {shards} shards: {out['autotune']}")
+    for i in range(warmup):
+        issue(i)
+    torch.cuda.synchronize()
+    st_ = ws.status(L)
+    assert (st_ == 0).all(), f"respond status {st_}"
+
+    def timed_launches(launch):
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(launches):
+            starts[k].record()
+            launch(k)
+            ends[k].record()
+        torch.cuda.synchronize()
+        wall_ = time.perf_counter() - t0
+        st = ws.status(L)
+        assert (st == 0).all(), f"respond status {st}"
+        return wall_, np.array([s_.elapsed_time(e) for s_, e in zip(starts, ends)])
+
+    # the stream launches of the timed region record events around their scan and chain kernels
+    # (no waits); the kernel durations of the roofline are these
+    if trace:
+        ws.trace(launches)
+    wall, lat = timed_launches(lambda k: issue(warmup + k))
+    if trace:
+        scan_ms, chain_ms = ws.kernel_times()
+        assert len(chain_ms) == launches, "a timed launch was not traced"
+        out["scan_ms"], out["chain_ms"] = float(np.mean(scan_ms)), float(np.mean(chain_ms))
+    out["stream"] = {"value": round(L * launches / wall, 2), "unit": "queries/s", "wall_s": wall,
+                     "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                                    "p99": round(float(np.percentile(lat, 99)), 3)}}
+    out["pipeline"], out["value"], out["wall_s"], out["latency"] = "stream", L * launches / wall, wall, lat
+    if graph:
+        eager_last = d_a_out.clone()
+        graphs = [g.RespondGraph(ctx, seat, ws, d_ids[warmup + k].data_ptr(), d_seeds[warmup + k].data_ptr(),
+                                 d_b[warmup + k].data_ptr(), L, d_a_out.data_ptr(), d_b_out.data_ptr(),
+                                 switch_modulus=Q36) for k in range(launches)]
+        for gr in graphs:
+            gr.launch(stream)
+        torch.cuda.synchronize()
+        d_a_out.zero_()
+        gwall, glat = timed_launches(lambda k: graphs[k].launch(stream))
+        assert torch.equal(d_a_out, eager_last), "graph replay differs from the stream launch"
+        for gr in graphs:
+            gr.close()
+        out["graph"] = {"value": round(L * launches / gwall, 2), "unit": "queries/s",
+                        "latency_ms": {"p50": round(float(np.percentile(glat, 50)), 3),
+                                       "p99": round(float(np.percentile(glat, 99)), 3)}}
+        # the faster of the two launch paths is the pipeline a server would run; both are reported
+        if gwall < wall:
+            out["pipeline"], out["value"], out["wall_s"], out["latency"] = "hipGraph", L * launches / gwall, gwall, glat
+    if single and M > 1:
+        # the same queries answered one B-query batch per launch (no merging across batches)
+        nb = min(launches * M, 32)
+        t_starts = [torch.cuda.Event(enable_timing=True) for _ in range(nb)]
+        t_ends = [torch.cuda.Event(enable_timing=True) for _ in range(nb)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(nb):
+            sl, off = warmup + k // M, (k % M) * B
+            t_starts[k].record()
+            ctx.respond_batch_device(seat, ws, d_ids[sl, off:].data_ptr(), d_seeds[sl, off:].data_ptr(),
+                                     d_b[sl, off:].data_ptr(), B, d_a_out.data_ptr(), d_b_out.data_ptr(),
+                                     stream=stream, switch_modulus=Q36)
+            t_ends[k].record()
+        torch.cuda.synchronize()
+        sw = time.perf_counter() - t0
+        slat = np.array([s_.elapsed_time(e) for s_, e in zip(t_starts, t_ends)])
+        out["one_batch_per_launch"] = {"value": round(B * nb / sw, 2), "unit": "queries/s", "batches": nb,
+                                       "latency_ms": {"p50": round(float(np.percentile(slat, 50)), 3),
+                                                      "p99": round(float(np.percentile(slat, 99)), 3)}}
+    ws.close()
+    return out
+
+
+def plain_leg(g, torch, dev, stream, ctx, seat, shards, count, rng):
+    """One seeded query per launch, waited for before the next (the plain drop-in)."""
+    wsp = g.Workspace(ctx, 1)
+    pids, pseeds, pbs = make_batches(rng, count, 1, shards, g.Q)
+    pd_ids = torch.from_numpy(pids.astype(np.int32)).to(dev)
+    pd_seeds = torch.from_numpy(pseeds).to(dev)
+    pd_b = torch.from_numpy(pbs.view(np.int64)).to(dev)
+    pa = torch.empty((1, N), dtype=torch.int64, device=dev)
+    pb = torch.empty((1, GAMMA), dtype=torch.int64, device=dev)
+
+    def one(i):
+        ctx.respond_batch_device(seat, wsp, pd_ids[i].data_ptr(), pd_seeds[i].data_ptr(), pd_b[i].data_ptr(), 1,
+                                 pa.data_ptr(), pb.data_ptr(), stream=stream, switch_modulus=Q36)
+        torch.cuda.synchronize()
+
+    for i in range(min(2, count)):
+        one(i)
+    lat = []
+    t0 = time.perf_counter()
+    for i in range(count):
+        t1 = time.perf_counter()
+        one(i)
+        lat.append((time.perf_counter() - t1) * 1e3)
+    qps = count / (time.perf_counter() - t0)
+    wsp.close()
+    return {"value": round(qps, 2), "unit": "queries/s", "queries": count,
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                           "p99": round(float(np.percentile(lat, 99)), 3)}}
+
+
+# 60-bit modular products of one exact-gadget query at gamma = 256 as the reference's algorithm
+# counts them (algorithmic work, not this kernel's own count): NTT(a) (11,264), step 1 (256 x
+# 2048), 255 recursion steps of one inverse transform (13,312 with its n^-1 scaling), two forward
+# transforms (2 x 11,264), the last digit's derivation (3 x 2048) and the key-switch rows
+# (3 x 2048), the online gather (255 x 3 x 2048), and the two final inverse transforms
+MM_PER_QUERY = 11264 + 256 * 2048 + 255 * (13312 + 2 * 11264 + 6 * 2048) + 255 * 3 * 2048 + 2 * 13312
+assert MM_PER_QUERY == 14_401_536
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=16, help="timed launches of the headline (512 queries each)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--merge", type=int, default=8, help="64-query batches merged into one launch")
+    ap.add_argument("--shards", type=int, default=512, help="headline shards (512 = 2^20 records of 512 B)")
+    ap.add_argument("--c2-shards", type=int, default=128, help="shards of the 2^18-record batched config")
+    ap.add_argument("--plain-shards", type=int, default=32, help="shards of the 2^16-record plain config")
+    ap.add_argument("--plain-queries", type=int, default=16)
     ap.add_argument("--no-served", action="store_true")
-    ap.add_argument("--no-graph", action="store_true", help="time stream launches instead of hipGraph replays")
+    ap.add_argument("--no-graph", action="store_true", help="time stream launches only")
     ap.add_argument("--no-autotune", action="store_true", help="keep --queries-per-cu instead of autotuning")
+    ap.add_argument("--no-extra", action="store_true", help="headline only: no 2^18, 2^16 or plain legs")
     ap.add_argument("--served-seconds", type=float, default=4.0)
     ap.add_argument("--queries-per-cu", type=int, default=2, help="launch shaping of the packing kernel (1, 2 or 3)")
-    ap.add_argument("--shards", type=int, default=128)
-    ap.add_argument("--plain-shards", type=int, default=32)
-    ap.add_argument("--plain-queries", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-rounds", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sm-tables", type=int, default=2048, help="special-modulus step-1 tables (2048 = 2^22 cells)")
     ap.add_argument("--sm-keys", type=int, default=256)
@@ -516,160 +749,62 @@ def main():
         print(json.dumps({"special_modulus": sm}), flush=True)
         return
 
+    y = rng.integers(0, g.Q, (3, N), dtype=np.uint64)
+    B, M = args.batch, args.merge
+    L = B * M
+
+    # --- headline: 2^20 x 512 B, 512 queries per launch
     log(f"bench: building {args.shards} shards of {N} x {ENTRY} B with stores")
     ctx = g.Context(g.params_d2048(), w_seed, GAMMA)
     ctx.set_queries_per_cu(args.queries_per_cu)
     t0 = time.perf_counter()
     build_db(g, ctx, args.shards, seed=1)
     build_s = time.perf_counter() - t0
-    y = rng.integers(0, g.Q, (3, N), dtype=np.uint64)
     seat = ctx.seat(y)
-    B = args.batch
-    M = args.merge                      # batches merged into one launch
-    L = B * M                           # queries per launch
-    ws = g.Workspace(ctx, L)
+    log("bench: headline launches")
+    head = batched_leg(g, torch, dev, stream, ctx, seat, args.shards, B, M, args.steps, args.warmup, rng,
+                       tune=not args.no_autotune, graph=not args.no_graph, single=not args.no_extra)
+    tuned = head.get("autotune")
+    plain = None
+    if not args.no_extra:
+        log("bench: plain one-query-per-launch path over the same database")
+        plain = plain_leg(g, torch, dev, stream, ctx, seat, args.shards, args.plain_queries, rng)
+    seat.close()
+    ctx.close()
+
+    c2 = c1 = None
+    if not args.no_extra:
+        log(f"bench: 2^18-record batched config ({args.c2_shards} shards)")
+        ctx2 = g.Context(g.params_d2048(), w_seed, GAMMA)
+        build_db(g, ctx2, args.c2_shards, seed=3)
+        ctx2.set_queries_per_cu(tuned["queries_per_cu"] if tuned else args.queries_per_cu)
+        seat2 = ctx2.seat(y)
+        c2 = batched_leg(g, torch, dev, stream, ctx2, seat2, args.c2_shards, B, M, min(args.steps, 12), 1, rng,
+                         tune=False, graph=False, single=True, trace=False)
+        seat2.close()
+        ctx2.close()
+        log(f"bench: 2^16-record plain config ({args.plain_shards} shards)")
+        ctx1 = g.Context(g.params_d2048(), w_seed, GAMMA)
+        build_db(g, ctx1, args.plain_shards, seed=2)
+        seat1 = ctx1.seat(y)
+        c1 = plain_leg(g, torch, dev, stream, ctx1, seat1, args.plain_shards, args.plain_queries, rng)
+        seat1.close()
+        ctx1.close()
+
+    qps = head["value"]
     launches = args.steps
-    total = args.warmup + launches
-    ids, seeds, bs = make_batches(rng, total * M, B, args.shards, g.Q)
-    d_ids = torch.from_numpy(ids.astype(np.int32)).to(dev).reshape(total, L)
-    d_seeds = torch.from_numpy(seeds).to(dev).reshape(total, L, 32)
-    d_b = torch.from_numpy(bs.view(np.int64)).to(dev).reshape(total, L, N)
-    d_a_out = torch.empty((L, N), dtype=torch.int64, device=dev)
-    d_b_out = torch.empty((L, GAMMA), dtype=torch.int64, device=dev)
-    torch.cuda.synchronize()
-
-    def issue(i):
-        ctx.respond_batch_device(seat, ws, d_ids[i].data_ptr(), d_seeds[i].data_ptr(), d_b[i].data_ptr(), L,
-                                 d_a_out.data_ptr(), d_b_out.data_ptr(), stream=stream, switch_modulus=Q36)
-
-    tuned = None
-    if not args.no_autotune:
-        # launch shape picked by timing both shapes on synthetic queries at this launch size
-        tuned = g.autotune(ctx, seat, ws, L)
-        log(f"bench: autotune at {L} queries/launch: {tuned}")
-
-    log("bench: warmup")
-    for i in range(args.warmup):
-        issue(i)
-    torch.cuda.synchronize()
-    st_ = ws.status(L)
-    assert (st_ == 0).all(), f"respond status {st_}"
-
-    def timed_launches(launch):
-        # `steps` launches back to back on one stream, each answering `merge` batches of B
-        # queries; a batch's latency is its launch's (events on the stream)
-        starts = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
-        ends = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for k in range(launches):
-            starts[k].record()
-            launch(k)
-            ends[k].record()
-        torch.cuda.synchronize()
-        wall_ = time.perf_counter() - t0
-        st = ws.status(L)
-        assert (st == 0).all(), f"respond status {st}"
-        return wall_, np.array([s_.elapsed_time(e) for s_, e in zip(starts, ends)])
-
-    # the stream launches of the timed region record events around their scan and chain
-    # kernels (no waits); the kernel durations of the roofline are these
-    ws.trace(launches)
-    eager_wall, eager_lat = timed_launches(lambda k: issue(args.warmup + k))
-    scan_ms, chain_ms = ws.kernel_times()
-    assert len(chain_ms) == launches, "a timed launch was not traced"
-    eager_last = d_a_out.clone()
-    pipeline, wall, lat = "stream", eager_wall, eager_lat
-    graph_wall = graph_lat = None
-    if not args.no_graph:
-        # the pipeline captured once per staged batch (kernel nodes only), replayed
-        log("bench: hipGraph-captured pipeline")
-        graphs = [g.RespondGraph(ctx, seat, ws, d_ids[args.warmup + k].data_ptr(),
-                                 d_seeds[args.warmup + k].data_ptr(), d_b[args.warmup + k].data_ptr(), L,
-                                 d_a_out.data_ptr(), d_b_out.data_ptr(), switch_modulus=Q36)
-                  for k in range(launches)]
-        for gr in graphs:
-            gr.launch(stream)
-        torch.cuda.synchronize()
-        d_a_out.zero_()
-        graph_wall, graph_lat = timed_launches(lambda k: graphs[k].launch(stream))
-        assert torch.equal(d_a_out, eager_last), "graph replay differs from the stream launch"
-        for gr in graphs:
-            gr.close()
-        # the faster of the two launch paths is the pipeline a server would run; both are reported
-        if graph_wall < eager_wall:
-            pipeline, wall, lat = "hipGraph", graph_wall, graph_lat
-    ms_per_step = wall * 1000 / (launches * M)
-    qps = L * launches / wall
-    eager_qps = L * launches / eager_wall
-    graph = None if graph_wall is None else {
-        "value": round(L * launches / graph_wall, 2), "unit": "queries/s",
-        "latency_ms": {"p50": round(float(np.percentile(graph_lat, 50)), 3),
-                       "p99": round(float(np.percentile(graph_lat, 99)), 3)}}
-
-    # the same queries answered one 64-query batch per launch (no merging across batches)
-    single = None
-    if M > 1:
-        log("bench: one batch per launch")
-        nb = min(launches * M, 32)
-        t_starts = [torch.cuda.Event(enable_timing=True) for _ in range(nb)]
-        t_ends = [torch.cuda.Event(enable_timing=True) for _ in range(nb)]
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for k in range(nb):
-            sl, off = args.warmup + k // M, (k % M) * B
-            t_starts[k].record()
-            ctx.respond_batch_device(seat, ws, d_ids[sl, off:].data_ptr(), d_seeds[sl, off:].data_ptr(),
-                                     d_b[sl, off:].data_ptr(), B, d_a_out.data_ptr(), d_b_out.data_ptr(),
-                                     stream=stream, switch_modulus=Q36)
-            t_ends[k].record()
-        torch.cuda.synchronize()
-        sw = time.perf_counter() - t0
-        slat = np.array([s_.elapsed_time(e) for s_, e in zip(t_starts, t_ends)])
-        single = {"value": round(B * nb / sw, 2), "unit": "queries/s", "batches": nb,
-                  "latency_ms": {"p50": round(float(np.percentile(slat, 50)), 3),
-                                 "p99": round(float(np.percentile(slat, 99)), 3)}}
-
-    scan_avg, chain_avg = float(np.mean(scan_ms)), float(np.mean(chain_ms))
-    # scan: algorithmic bytes per query = the shard's records (d x entry) + its b (d x 8)
-    # + the tag key (d x 8); per launch x B queries
+    scan_avg, chain_avg = head["scan_ms"], head["chain_ms"]
+    # scan: algorithmic bytes per query = the shard's records (d x entry) + its b (d x 8) + the
+    # tag key (d x 8); per launch x L queries
     scan_bytes = L * (N * ENTRY + N * 8 + N * 8)
     scan_gbs = scan_bytes / (scan_avg * 1e-3) / 1e9
-    # chain: 60-bit modular products per query (exact gadget, gamma = 256):
-    # 255 steps x (3 transforms x 11264 + 10 x 2048 pointwise) + step 1 and the final transforms
-    mm_per_query = 255 * (3 * 11264 + 10 * 2048) + 2048 + 3 * 11264
-    chain_gmms = L * mm_per_query / (chain_avg * 1e-3) / 1e9
-    store_bytes = L * GAMMA * N * 16
-    chain_gbs = store_bytes / (chain_avg * 1e-3) / 1e9
+    chain_gmms = L * MM_PER_QUERY / (chain_avg * 1e-3) / 1e9
+    chain_gbs = L * GAMMA * N * 16 / (chain_avg * 1e-3) / 1e9
 
-    # plain path: one query per launch over the 2^16-record database
-    log("bench: plain one-query-per-launch path")
-    ctxp = g.Context(g.params_d2048(), w_seed, GAMMA)
-    build_db(g, ctxp, args.plain_shards, seed=2)
-    seatp = ctxp.seat(y)
-    wsp = g.Workspace(ctxp, 1)
-    P = args.plain_queries
-    pids, pseeds, pbs = make_batches(rng, P, 1, args.plain_shards, g.Q)
-    pd_ids = torch.from_numpy(pids.astype(np.int32)).to(dev)
-    pd_seeds = torch.from_numpy(pseeds).to(dev)
-    pd_b = torch.from_numpy(pbs.view(np.int64)).to(dev)
-    pa = torch.empty((1, N), dtype=torch.int64, device=dev)
-    pb = torch.empty((1, GAMMA), dtype=torch.int64, device=dev)
-    for i in range(min(2, P)):
-        ctxp.respond_batch_device(seatp, wsp, pd_ids[i].data_ptr(), pd_seeds[i].data_ptr(), pd_b[i].data_ptr(), 1,
-                                  pa.data_ptr(), pb.data_ptr(), stream=stream, switch_modulus=Q36)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(P):
-        ctxp.respond_batch_device(seatp, wsp, pd_ids[i].data_ptr(), pd_seeds[i].data_ptr(), pd_b[i].data_ptr(), 1,
-                                  pa.data_ptr(), pb.data_ptr(), stream=stream, switch_modulus=Q36)
-        torch.cuda.synchronize()
-    plain_qps = P / (time.perf_counter() - t0)
-
-    # HBM bytes per launch measured by rocprofv3 counters for this workload (512 queries
-    # per launch), committed beside the profile; scaled to this run's launch size
+    # HBM bytes per launch measured by rocprofv3 counters for this workload, committed beside the
+    # profile; scaled to this run's launch size
     traffic_chain = traffic_scan = None
-    tpath = os.path.join(ROOT, "profiles", "respond_batched512_d2048_g256_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "respond_batched512_d2048_g256_2p20_traffic.json")
     if os.path.exists(tpath):
         tj = json.load(open(tpath))
         traffic_chain = round(tj["k_chain_512_queries_bytes"] * L / 512)
@@ -690,51 +825,60 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline")
-        cpu = cpu_baseline(args.cpu_seconds)
+        cpu = cpu_baseline(args.cpu_seconds, rounds=args.cpu_rounds)
         if sm is not None:
             sm["cpu_baseline"] = sm_cpu_baseline(min(args.cpu_seconds, 10.0))
 
+    lat = head["latency"]
     result = {
         "metric": "respond queries/s + DB GB/s (byte-exact) vs plain GPU; p50/p99 latency",
         "value": round(qps, 2),
         "unit": "queries/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": launches,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(head["wall_s"] * 1000 / launches, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64 (exact mod 2^60-2^14+1, on 32-bit lanes)",
         "data": "synthetic: uniform random records, seeds, query b and key rows from fixed seeds",
         "config": {
-            "workload": f"batched TwoPacking respond, d2048, 2^18 records x 512 B ({args.shards} shards, gamma 256), "
-                        f"stores precomputed, {B} seeded queries per batch under one seat, uniform indices, "
-                        f"36-bit modulus switch, {M} batches merged per launch ({L} queries), launches back to back",
-            "merge": M,
-            "queries_per_launch": L,
-            "pipeline": pipeline,
-            "queries_per_cu": tuned["queries_per_cu"] if tuned else args.queries_per_cu,
-            "autotune": tuned,
-            "global_batch": B,
+            "workload": f"batched TwoPacking respond, d2048 exact gadget, 2^20 records x 512 B ({args.shards} shards, "
+                        f"gamma 256), stores precomputed, {B} seeded queries per batch under one seat, uniform "
+                        f"indices, 36-bit modulus switch; a step is one launch of {M} merged batches ({L} queries), "
+                        f"launches back to back",
             "records": args.shards * N,
             "record_bytes": ENTRY,
+            "queries_per_step": L,
+            "merge": M,
+            "global_batch": B,
+            "pipeline": head["pipeline"],
+            "queries_per_cu": tuned["queries_per_cu"] if tuned else args.queries_per_cu,
+            "autotune": tuned,
         },
         "db_gbs": round(qps * N * ENTRY / 1e9, 3),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
-                       "p99": round(float(np.percentile(lat, 99)), 3)},
-        "one_batch_per_launch": single,
-        "graph_replays": graph,
-        "stream_launches": {"value": round(eager_qps, 2), "unit": "queries/s",
-                            "latency_ms": {"p50": round(float(np.percentile(eager_lat, 50)), 3),
-                                           "p99": round(float(np.percentile(eager_lat, 99)), 3)}},
-        "plain_gpu": {"value": round(plain_qps, 2), "unit": "queries/s",
-                      "workload": "one seeded query per launch, 2^16 records x 512 B (32 shards)"},
-        "vs_plain_gpu": round(qps / plain_qps, 3),
+                       "p99": round(float(np.percentile(lat, 99)), 3),
+                       "unit": "one launch (512 queries), events on the stream"},
+        "one_batch_per_launch": head.get("one_batch_per_launch"),
+        "graph_replays": head.get("graph"),
+        "stream_launches": head["stream"],
+        "plain_gpu": None if plain is None else dict(plain, workload="one seeded query per launch over the same "
+                                                                      "2^20-record database, waited for in turn"),
+        "vs_plain_gpu": None if plain is None else round(qps / plain["value"], 3),
+        "batched_2p18": None if c2 is None else {
+            "workload": f"2^18 records x 512 B ({args.c2_shards} shards), {L} queries per launch and one "
+                        f"{B}-query batch per launch, stream launches",
+            "value": c2["stream"]["value"], "unit": "queries/s", "latency_ms": c2["stream"]["latency_ms"],
+            "one_batch_per_launch": c2.get("one_batch_per_launch")},
+        "plain_2p16": None if c1 is None else dict(c1, workload=f"one seeded query per launch, 2^16 records x 512 B "
+                                                                 f"({args.plain_shards} shards)"),
         "roofline": {"bound": "valu", "kernel": "k_chain",
                      "achieved": round(chain_gmms * 4 / 1e3, 3), "peak": VALU_MAD_PEAK_T,
-                     "unit": "T mad32/s", "frac": round(chain_gmms * 4 / 1e3 / VALU_MAD_PEAK_T, 4), "traffic": traffic_chain,
-                     "launch_ms": round(chain_avg, 4), "mm_per_query": mm_per_query,
+                     "unit": "T mad32/s", "frac": round(chain_gmms * 4 / 1e3 / VALU_MAD_PEAK_T, 4),
+                     "traffic": traffic_chain,
+                     "launch_ms": round(chain_avg, 4), "mm_per_query": MM_PER_QUERY,
                      "timing": "HIP events on the launch stream around this kernel in each launch of the timed region",
                      "modmul60_per_s": round(chain_gmms * 1e9, 0),
                      "definition": "4 v_mad_u64_u32 (32x32->64) per 60-bit modular product, algorithmic; peak = "
@@ -742,7 +886,9 @@ def main():
         "roofline_scan": {"bound": "hbm", "kernel": "k_scan", "achieved": round(scan_gbs, 2),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(scan_gbs / HBM_PEAK_GBS, 4),
                           "traffic": traffic_scan, "launch_ms": round(scan_avg, 4),
-                          "timing": "HIP events on the launch stream around this kernel in each launch of the timed region"},
+                          "algorithmic_bytes": scan_bytes,
+                          "timing": "HIP events on the launch stream around this kernel in each launch of the timed "
+                                    "region, over the 2^20-record database (512 MiB of records)"},
         "store_stream_gbs": round(chain_gbs, 2),
         "build_s": round(build_s, 2),
         "served": serve,
