@@ -143,8 +143,8 @@ int inspire_respond_batch_device(inspire_ctx *ctx, const inspire_seat *seat, ins
  * before the call (their fence events), not for the whole device; a launch enqueued afterwards
  * that names a destroyed shard gets status -1. inspire_respond_batch_device_noalloc records no
  * fence event: nothing it reads may be destroyed while it runs. */
-/* inspire_respond_batch_device, then waits: kernel_ms[0] receives the DB-scan kernel's
- * duration and kernel_ms[1] the recursion kernel's, from HIP events recorded on
+/* inspire_respond_batch_device, then waits: kernel_ms[0] receives the DB-scan stage's
+ * duration (grouping included) and kernel_ms[1] the recursion kernel's, from HIP events recorded on
  * `stream` around each launch. For measurement. */
 int inspire_respond_batch_device_timed(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws,
                                        const uint32_t *d_shard_ids, const uint8_t *d_seeds, const uint64_t *d_b,
@@ -169,13 +169,14 @@ int inspire_respond_batch_device_noalloc(inspire_ctx *ctx, const inspire_seat *s
                                          int32_t *d_status, void *stream, char *err, size_t err_cap);
 /* Kernel trace (measurement; no counterpart in the reference): arms ws for its next
  * `launches` device-buffer launches (at most 2^20; 0 disarms), each of which then records HIP
- * events on its own stream around the DB-scan and recursion kernels without waiting, so a
- * timed loop keeps its pipelining. inspire_workspace_kernel_times waits for the last traced
- * launch, writes the durations (ms) of up to cap of them to scan_ms and chain_ms, their
- * number to *count, and disarms. Like every use of a workspace, one caller at a time. */
+ * events on its own stream around its shard-grouping, DB-scan and recursion kernels without
+ * waiting, so a timed loop keeps its pipelining. inspire_workspace_kernel_times waits for every
+ * traced launch, writes the durations (ms) of up to cap of them to group_ms (0 for a launch
+ * that is not grouped), scan_ms and chain_ms, their number to *count, and disarms. Like every
+ * use of a workspace, one caller at a time. */
 int inspire_workspace_trace(inspire_workspace *ws, size_t launches, char *err, size_t err_cap);
-int inspire_workspace_kernel_times(inspire_workspace *ws, float *scan_ms, float *chain_ms, size_t cap,
-                                   size_t *count, char *err, size_t err_cap);
+int inspire_workspace_kernel_times(inspire_workspace *ws, float *group_ms, float *scan_ms, float *chain_ms,
+                                   size_t cap, size_t *count, char *err, size_t err_cap);
 /* Copies count statuses (0 = answered, c+1 = column c failed its store tag, -1 = the id names
  * no shard with a store). */
 int inspire_workspace_status(inspire_workspace *ws, size_t count, int32_t *status, char *err, size_t err_cap);

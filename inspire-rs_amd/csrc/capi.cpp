@@ -496,6 +496,7 @@ static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_wo
     b.b_out = d_b_prefix_out;
     b.status = ws->status.as<int32_t>();
     b.switch_modulus = switch_modulus;
+    ws_group(ws, b);
     if (d_wire) {
         const WireLayout wl = wire_layout((uint32_t)ctx->n, (uint32_t)ctx->gamma, switch_modulus ? switch_modulus : kQ);
         if (wire_stride % 4 || wire_stride < ((wl.total + 3) & ~3u) || reinterpret_cast<uintptr_t>(d_wire) % 4) {
@@ -521,12 +522,15 @@ static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_wo
     if (queries_per_cu) dev.queries_per_cu = queries_per_cu;  // a trial shape (autotune)
     if (!timing) {
         if (hipEvent_t *tr = ws->trace.next()) {
-            // a traced launch: events around both kernels, read later, no wait
+            // a traced launch: events around the grouping, scan and packing kernels, read later,
+            // no wait
             hipError_t e = hipEventRecord(tr[0], st);
-            if (e == hipSuccess) e = launch_scan_only(dev, b, st);
+            if (e == hipSuccess) e = launch_group_only(dev, b, st);
             if (e == hipSuccess) e = hipEventRecord(tr[1], st);
-            if (e == hipSuccess) e = launch_chain_only(dev, b, st);
+            if (e == hipSuccess) e = launch_scan_only(dev, b, st);
             if (e == hipSuccess) e = hipEventRecord(tr[2], st);
+            if (e == hipSuccess) e = launch_chain_only(dev, b, st);
+            if (e == hipSuccess) e = hipEventRecord(tr[3], st);
             HIP_TRY(e);
             ws->trace.done();
         } else {
@@ -539,6 +543,7 @@ static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_wo
     hipEvent_t ev[3];
     for (int i = 0; i < 3; i++) HIP_TRY(hipEventCreate(&ev[i]));
     hipError_t e = hipEventRecord(ev[0], st);
+    if (e == hipSuccess) e = launch_group_only(dev, b, st);
     if (e == hipSuccess) e = launch_scan_only(dev, b, st);
     if (e == hipSuccess) e = hipEventRecord(ev[1], st);
     if (e == hipSuccess) e = launch_chain_only(dev, b, st);
@@ -630,15 +635,18 @@ int inspire_workspace_trace(inspire_workspace *ws, size_t launches, char *err, s
         set_err(err, err_cap, "a trace holds at most %u launches, asked for %zu", 1u << 20, launches);
         return INSPIRE_ERR_INVALID;
     }
-    HIP_TRY(ws->trace.arm(launches, 2));
+    HIP_TRY(ws->trace.arm(launches, 3));
     return INSPIRE_OK;
 }
 
-int inspire_workspace_kernel_times(inspire_workspace *ws, float *scan_ms, float *chain_ms, size_t cap,
-                                   size_t *count, char *err, size_t err_cap) {
-    if (!ws || !count || (cap && (!scan_ms || !chain_ms))) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
-    float *const out[2] = {scan_ms, chain_ms};
-    HIP_TRY(ws->trace.read(out, 2, cap, count));
+int inspire_workspace_kernel_times(inspire_workspace *ws, float *group_ms, float *scan_ms, float *chain_ms,
+                                   size_t cap, size_t *count, char *err, size_t err_cap) {
+    if (!ws || !count || (cap && (!group_ms || !scan_ms || !chain_ms))) {
+        set_err(err, err_cap, "null argument");
+        return INSPIRE_ERR_INVALID;
+    }
+    float *const out[3] = {group_ms, scan_ms, chain_ms};
+    HIP_TRY(ws->trace.read(out, 3, cap, count));
     return INSPIRE_OK;
 }
 
@@ -727,6 +735,7 @@ int inspire_respond_graph_create(inspire_ctx *ctx, const inspire_seat *seat, ins
     b.b_out = d_b_prefix_out;
     b.status = ws->status.as<int32_t>();
     b.switch_modulus = switch_modulus;
+    ws_group(ws, b);
     inspire_graph *g = new (std::nothrow) inspire_graph();
     if (!g) { set_err(err, err_cap, "out of host memory"); return INSPIRE_ERR_DEVICE; }
     g->ctx = ctx;
@@ -901,6 +910,7 @@ static int respond_host(inspire_ctx *ctx, const inspire_seat *seat, const uint32
     bd.b_out = ws->b_out.as<uint64_t>();
     bd.status = ws->status.as<int32_t>();
     bd.switch_modulus = switch_modulus;
+    ws_group(ws, bd);
     const WireLayout wl = wire_layout((uint32_t)n, (uint32_t)g, switch_modulus ? switch_modulus : kQ);
     const size_t dstride = (wl.total + 3) & ~(size_t)3;
     if (wire_out) {

@@ -305,7 +305,9 @@ void k_chain(CtxDev c, BatchDev bt) {
     int &rejected = reinterpret_cast<int *>(C)[1];
     auto aix = [](int e) { return e; };
 
-    const int qi = blockIdx.x, tid = threadIdx.x, gamma = c.gamma;
+    // a grouped launch takes its queries in shard order, a contiguous slice of it per XCD
+    const int qi = batch_grouped(c, bt) ? (int)bt.order[xcd_slot(blockIdx.x, bt.count)] : (int)blockIdx.x;
+    const int tid = threadIdx.x, gamma = c.gamma;
     // the ids of device-buffer batches are not checked on the host: a query naming no shard
     // with a store is refused here, before anything reads the shard (the whole workgroup)
     // One thread reads the entry and the workgroup takes its copy from LDS, so every wave
@@ -336,9 +338,10 @@ void k_chain(CtxDev c, BatchDev bt) {
     // b_poly[j] is parked in the first word of the query's own scan partials for column j,
     // which only this thread reads, until the epilogue (keeps it out of LDS)
     uint64_t *const bpoly = bt.scan_part + (size_t)qi * bt.ks * gamma * 4;
+    const int slices = scan_slices(c, bt);
     for (int j = tid; j < gamma; j += T) {
         uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-        for (int s = 0; s < bt.ks; s++) {
+        for (int s = 0; s < slices; s++) {
             const uint64_t *p = bt.scan_part + (((size_t)qi * bt.ks + s) * gamma + j) * 4;
             s0 += p[0]; s1 += p[1]; s2 += p[2]; s3 += p[3];
         }

@@ -69,12 +69,54 @@ struct BatchDev {
     uint64_t *b_out;            // count x gamma
     int32_t *status;            // count: 0 ok, c+1 = column c failed its tag, -1 = no shard with a store
     uint64_t switch_modulus;    // 0 = no modulus switch
+    // Shard grouping of the launch (a workspace's or a serving slot's scratch, count <= kGroupMax;
+    // all null for an ungrouped launch): k_group writes the query indices sorted by shard id
+    // (order, count words), the start of each shard's run in that order (first, count + 1
+    // words) and the number of runs (ngroups). The DB scan then reads each distinct shard once
+    // for all of its queries, and the packing kernel takes its queries in that order, a
+    // contiguous slice of it per XCD, so chains of one shard share their store rows in one L2.
+    uint32_t *order = nullptr, *first = nullptr, *ngroups = nullptr;
     // when non-null, each query's ServerResponse bincode bytes at wire_out + qi * wire_stride
     // (4-byte aligned, stride a multiple of 4 holding the length rounded up to 4); a_out and
     // b_out may then be null
     uint8_t *wire_out = nullptr;
     size_t wire_stride = 0;
 };
+
+// Largest launch the grouping kernel sorts (one workgroup, keys in LDS); larger launches scan
+// query by query.
+constexpr int kGroupMax = 4096;
+// XCDs of an MI355X: consecutive workgroups of a launch are dispatched to them in turn.
+constexpr int kXcds = 8;
+
+// Position in the shard-sorted order of the query that workgroup b of `count` takes: XCD x
+// (= b mod 8) gets the x-th contiguous slice of the order, as many positions as it gets
+// workgroups, in dispatch order (a bijection of [0, count)).
+__host__ __device__ inline int xcd_slot(int b, int count) {
+    const int x = b % kXcds, j = b / kXcds, m = count / kXcds, r = count % kXcds;
+    return x * m + (x < r ? x : r) + j;
+}
+
+// Whether a launch is grouped by shard: it carries the group buffers, fits the sort, and its
+// shards are wide enough for the grouped scan's lane layout. The scan and packing kernels both
+// decide by this.
+__host__ __device__ inline bool batch_grouped(const CtxDev &c, const BatchDev &b) {
+    return b.order && b.first && b.ngroups && b.count <= kGroupMax && c.gamma >= 4;
+}
+
+// Workgroup size of the DB scans over 4-column groups (gamma >= 4): one lane per (row split,
+// group), at most 256.
+__host__ __device__ inline int scan_threads(int gamma, int ks) {
+    const int G = gamma / 4;
+    return G * ks < 256 ? G * ks : 256;
+}
+// Partial slices per query the scan leaves in scan_part (each query's stride stays ks slices):
+// ks for a query-by-query scan; one per workgroup of row splits for a grouped one, which adds
+// its splits in LDS.
+__host__ __device__ inline int scan_slices(const CtxDev &c, const BatchDev &b) {
+    if (!batch_grouped(c, b)) return b.ks;
+    return b.ks / (scan_threads(c.gamma, b.ks) / (c.gamma / 4));
+}
 
 // Layout of a packed InspiRING ServerResponse in bincode (respond.rs:111-176, poly.rs:166-225,
 // 286-310): u32 variant 1 | u64 A | a tight (A bytes) | u64 1 | u64 modulus | u64 modulus |

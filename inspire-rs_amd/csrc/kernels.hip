@@ -208,6 +208,211 @@ __global__ __launch_bounds__(kThreads) void k_scan(CtxDev c, BatchDev bt) {
         for (int v = 0; v < 4; v++) o[u * 4 + v] = acc[u][v];
 }
 
+// Groups a launch's queries by shard (count <= kGroupMax, one workgroup of 1024 threads): a
+// bitonic sort of the keys (shard id, query index) in LDS, then the scan tasks: each shard's run
+// of queries in that order cut into tasks of two (the last of an odd run takes one). first[t]
+// holds task t's start position | its length << 16, *ngroups the number of tasks.
+__global__ __launch_bounds__(1024) void k_group(const uint32_t *ids, int count, uint32_t *order, uint32_t *first,
+                                                uint32_t *ngroups) {
+    constexpr int T = 1024, PER = kGroupMax / T;
+    __shared__ uint64_t key[kGroupMax];
+    __shared__ int part[T];
+    const int t = threadIdx.x;
+    int P = 1;
+    while (P < count) P <<= 1;
+    for (int i = t; i < P; i += T) key[i] = i < count ? ((uint64_t)ids[i] << 32 | (uint32_t)i) : ~0ULL;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = t; i < P; i += T) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint64_t a = key[i], b = key[l];
+                    if ((a > b) == ((i & k) == 0)) {
+                        key[i] = b;
+                        key[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    // thread t owns positions PER t .. PER t + PER - 1; a run starts where the shard changes
+    auto run_start = [&](int i) { return i == 0 || (key[i] >> 32) != (key[i - 1] >> 32); };
+    int last = -1;
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int i = t * PER + u;
+        if (i < count) {
+            order[i] = (uint32_t)key[i];
+            if (run_start(i)) last = i;
+        }
+    }
+    // inclusive max-scan of the last run start up to each thread's range
+    part[t] = last;
+    __syncthreads();
+    for (int off = 1; off < T; off <<= 1) {
+        const int v = t >= off ? part[t - off] : -1;
+        __syncthreads();
+        if (v > part[t]) part[t] = v;
+        __syncthreads();
+    }
+    int rs = t > 0 ? part[t - 1] : 0;  // start of the run the thread's first position is in
+    __syncthreads();
+    uint32_t task[PER];
+    int cnt = 0;
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int i = t * PER + u;
+        task[u] = 0;
+        if (i < count) {
+            if (run_start(i)) rs = i;
+            if (((i - rs) & 1) == 0) {
+                const int len = i + 1 < count && !run_start(i + 1) ? 2 : 1;
+                task[u] = (uint32_t)i | (uint32_t)len << 16 | 0x80000000u;
+                cnt++;
+            }
+        }
+    }
+    part[t] = cnt;
+    __syncthreads();
+    for (int off = 1; off < T; off <<= 1) {  // inclusive prefix sum of the per-thread task counts
+        const int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int ti = part[t] - cnt;
+#pragma unroll
+    for (int u = 0; u < PER; u++)
+        if (task[u]) first[ti++] = task[u] & 0x7fffffffu;
+    if (t == T - 1) *ngroups = (uint32_t)part[T - 1];
+}
+
+// One pass of the grouped scan over a lane's rows: the b fold of NQ queries and the column tag
+// (the same for every query of the shard), accumulated in registers.
+template <int NQ, int RF>
+__device__ __forceinline__ void grouped_pass(const uint64_t *rec, int G, const uint64_t *tag_key,
+                                             const uint64_t *b0, const uint64_t *b1, int n, int r0, int rows,
+                                             uint64_t acc[4][2][2], uint64_t tag[4][2]) {
+    const uint64_t *const b[2] = {b0, b1};
+    for (int k0 = r0; k0 < r0 + rows; k0 += RF) {  // RF rows in flight
+        uint64_t h4[RF], kv[RF], bv[NQ][RF];
+#pragma unroll
+        for (int r = 0; r < RF; r++) {
+            const int k = k0 + r;
+            h4[r] = rec[(size_t)k * G];
+            kv[r] = tag_key[k];
+#pragma unroll
+            for (int v = 0; v < NQ; v++) bv[v][r] = b[v][(n - k) & (n - 1)];
+        }
+#pragma unroll
+        for (int r = 0; r < RF; r++) {
+            const int k = k0 + r;
+            uint32_t bl[NQ], bh[NQ];
+#pragma unroll
+            for (int v = 0; v < NQ; v++) {
+                // brev[0] = b[0], brev[k] = -b[n-k]
+                const uint64_t bk = (k != 0 && bv[v][r] != 0) ? kQ - bv[v][r] : bv[v][r];
+                bl[v] = (uint32_t)bk;
+                bh[v] = (uint32_t)(bk >> 32);
+            }
+            const uint32_t kl = (uint32_t)kv[r], kh = (uint32_t)(kv[r] >> 32);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t h = (uint32_t)(h4[r] >> (16 * u)) & 0xffffu;
+#pragma unroll
+                for (int v = 0; v < NQ; v++) {
+                    acc[u][v][0] += (uint64_t)h * bl[v];
+                    acc[u][v][1] += (uint64_t)h * bh[v];
+                }
+                tag[u][0] += (uint64_t)h * kl;
+                tag[u][1] += (uint64_t)h * kh;
+            }
+        }
+    }
+}
+
+// DB scan of a grouped launch: one workgroup per (task, block of row splits), lanes laid out as
+// in k_scan. A task is one or two queries of one shard (k_group), so each record word is read
+// from HBM once per task and folded against the b of both queries, and the column tag, the same
+// for every query of the shard, is summed once per task. The workgroup then adds its row splits'
+// partials in LDS and writes one slice per query (scan_slices), a quarter of k_scan's partials
+// at ring 2048, so the packing kernel reads fewer. Its per-query tag check is unchanged.
+template <bool UNIFORM_ROW>
+__device__ __forceinline__ void scan_grouped_body(const CtxDev &c, const BatchDev &bt) {
+    // per lane: 4 columns x 2 halves of the tag sums, then of one query's fold sums (16 KiB)
+    __shared__ uint64_t red[kThreads * 8];
+    const int task = blockIdx.x;
+    if (task >= (int)*bt.ngroups) return;
+    const int n = c.n, gamma = c.gamma;
+    const int G = gamma / 4;
+    const int threads = blockDim.x;
+    const int per_block = threads / G;
+    const int g = threadIdx.x % G, w = threadIdx.x / G;
+    int split = blockIdx.y * per_block + w;
+    if (UNIFORM_ROW) split = __builtin_amdgcn_readfirstlane(split);
+    // the task's queries are the same on every lane: scalar registers
+    const uint32_t tw = __builtin_amdgcn_readfirstlane(bt.first[task]);
+    const int p0 = (int)(tw & 0xffffu), nq = (int)(tw >> 16);
+    uint32_t qi[2];
+    qi[0] = __builtin_amdgcn_readfirstlane(bt.order[p0]);
+    qi[1] = __builtin_amdgcn_readfirstlane(bt.order[nq == 2 ? p0 + 1 : p0]);
+    // (indices made scalar, pointers derived from the kernel arguments: so the loads of b and of
+    // the tag key stay scalar loads)
+    const uint32_t sid = __builtin_amdgcn_readfirstlane(bt.shard_ids[qi[0]]);
+    const ShardDev sh = shard_entry(c.shards, c.num_shards, sid);
+    if (!shard_ready(sh)) return;
+    const int rows = n / bt.ks, r0 = split * rows;
+    const uint64_t *rec = reinterpret_cast<const uint64_t *>(sh.records) + g;
+    const uint64_t *b0 = bt.b + (size_t)qi[0] * n, *b1 = bt.b + (size_t)qi[1] * n;
+    uint64_t acc[4][2][2], tag[4][2];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        tag[u][0] = tag[u][1] = 0;
+        acc[u][0][0] = acc[u][0][1] = acc[u][1][0] = acc[u][1][1] = 0;
+    }
+    if (nq == 2)
+        grouped_pass<2, 2>(rec, G, sh.tag_key, b0, b1, n, r0, rows, acc, tag);
+    else
+        grouped_pass<1, 8>(rec, G, sh.tag_key, b0, b1, n, r0, rows, acc, tag);
+    // add the workgroup's row splits in LDS and write one slice per query: the tag sums once
+    // (the same for both queries), then each query's fold sums. A slice holds, per column j,
+    // (fold lo, fold hi, tag lo, tag hi) at 4 j.
+    const int slice = blockIdx.y, outs = G * 8;
+    uint64_t *mine = red + (size_t)(w * G + g) * 8;
+    uint64_t *o0 = bt.scan_part + ((size_t)qi[0] * bt.ks + slice) * gamma * 4;
+    uint64_t *o1 = bt.scan_part + ((size_t)qi[1] * bt.ks + slice) * gamma * 4;
+    for (int round = 0; round <= nq; round++) {  // round 0: tags, round 1 + v: query v's folds
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            mine[2 * u + 0] = round == 0 ? tag[u][0] : round == 1 ? acc[u][0][0] : acc[u][1][0];
+            mine[2 * u + 1] = round == 0 ? tag[u][1] : round == 1 ? acc[u][0][1] : acc[u][1][1];
+        }
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < outs; idx += threads) {
+            uint64_t sum = 0;
+            for (int ww = 0; ww < per_block; ww++) sum += red[(size_t)ww * G * 8 + idx];
+            // idx = 8 g' + 2 u + e: column 4 g' + u, entry e of the fold or 2 + e of the tag
+            const int at = (idx >> 1) * 4 + (idx & 1);
+            if (round == 0) {
+                o0[at + 2] = sum;
+                if (nq == 2) o1[at + 2] = sum;
+            } else {
+                (round == 1 ? o0 : o1)[at] = sum;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Wide shards (64 or more column groups, the row words scalar): registers budgeted for six
+// waves per SIMD, so a 2^20-record launch's workgroups run in one round.
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8)))
+void k_scan_grouped(CtxDev c, BatchDev bt) { scan_grouped_body<true>(c, bt); }
+__global__ __launch_bounds__(kThreads) void k_scan_grouped_narrow(CtxDev c, BatchDev bt) {
+    scan_grouped_body<false>(c, bt);
+}
+
 // Narrow shards (gamma < 4 columns): one column per lane.
 __global__ __launch_bounds__(kThreads) void k_scan_narrow(CtxDev c, BatchDev bt) {
     const int qi = blockIdx.x, split = blockIdx.y;
@@ -245,8 +450,28 @@ __global__ __launch_bounds__(kThreads) void k_scan_narrow(CtxDev c, BatchDev bt)
 // ---------------------------------------------------------------------------
 // launchers
 
+
+hipError_t launch_group_only(const CtxDev &c, const BatchDev &b, hipStream_t s) {
+    if (b.count == 0 || !batch_grouped(c, b)) return hipSuccess;
+    hipLaunchKernelGGL(k_group, dim3(1), dim3(1024), 0, s, b.shard_ids, b.count, b.order, b.first, b.ngroups);
+    return hipGetLastError();
+}
+
+// The DB scan of a launch whose grouping (launch_group_only) is already enqueued.
 hipError_t launch_scan_only(const CtxDev &c, const BatchDev &b, hipStream_t s) {
     if (b.count == 0) return hipSuccess;
+    if (batch_grouped(c, b)) {
+        const int G = c.gamma / 4;
+        const int threads = scan_threads(c.gamma, b.ks);
+        // one workgroup row per possible task (at most one per query); the tasks past the count
+        // k_group found exit at once
+        if (G >= 64)
+            hipLaunchKernelGGL(k_scan_grouped, dim3(b.count, b.ks / (threads / G)), dim3(threads), 0, s, c, b);
+        else
+            hipLaunchKernelGGL(k_scan_grouped_narrow, dim3(b.count, b.ks / (threads / G)), dim3(threads), 0, s, c,
+                               b);
+        return hipGetLastError();
+    }
     if (c.gamma >= 4) {
         // one lane per (split, group): a block holds whole splits, the grid the rest
         const int G = c.gamma / 4;
@@ -263,7 +488,8 @@ hipError_t launch_scan_only(const CtxDev &c, const BatchDev &b, hipStream_t s) {
 
 hipError_t launch_respond(const CtxDev &c, const BatchDev &b, hipStream_t s) {
     if (b.count == 0) return hipSuccess;
-    hipError_t e = launch_scan_only(c, b, s);
+    hipError_t e = launch_group_only(c, b, s);
+    if (e == hipSuccess) e = launch_scan_only(c, b, s);
     if (e != hipSuccess) return e;
     return launch_chain_only(c, b, s);
 }

@@ -7,6 +7,8 @@
 namespace inspire {
 
 hipError_t launch_respond(const CtxDev &c, const BatchDev &b, hipStream_t s);
+// the three stages of launch_respond: shard grouping (grouped launches only), DB scan, packing
+hipError_t launch_group_only(const CtxDev &c, const BatchDev &b, hipStream_t s);
 hipError_t launch_scan_only(const CtxDev &c, const BatchDev &b, hipStream_t s);
 hipError_t launch_chain_only(const CtxDev &c, const BatchDev &b, hipStream_t s);
 hipError_t launch_expand_mask(int logn, int rows, const uint8_t *seed, uint64_t stream, int dropped,

@@ -118,6 +118,7 @@ struct inspire_workspace {
     inspire_ctx *ctx = nullptr;
     size_t max_count = 0;
     DevBuf scan_part, status;
+    DevBuf group;  // shard grouping of a launch: order, first, ngroups (up to kGroupMax queries)
     // staging for the host-buffer entry point
     DevBuf ids, seeds, a_in, b, a_out, b_out, wire;
     size_t wire_cap = 0;
@@ -204,8 +205,20 @@ inline int ws_reserve(inspire_workspace *ws, size_t count, char *err, size_t err
     size_t c = count < 1 ? 1 : count;
     HIP_TRY(ws->scan_part.alloc(c * ctx->ks() * ctx->gamma * 4 * sizeof(uint64_t)));
     HIP_TRY(ws->status.alloc(c * sizeof(int32_t)));
+    const size_t gc = c < (size_t)kGroupMax ? c : (size_t)kGroupMax;
+    HIP_TRY(ws->group.alloc((2 * gc + 2) * sizeof(uint32_t)));
     ws->max_count = c;
     return INSPIRE_OK;
+}
+
+// Points a launch of count queries at the workspace's grouping buffers when it can be grouped.
+inline void ws_group(const inspire_workspace *ws, BatchDev &b) {
+    const size_t gc = ws->max_count < (size_t)kGroupMax ? ws->max_count : (size_t)kGroupMax;
+    if ((size_t)b.count > gc || !ws->group.p) return;
+    uint32_t *g = ws->group.as<uint32_t>();
+    b.order = g;
+    b.first = g + gc;
+    b.ngroups = g + 2 * gc + 1;
 }
 
 }  // namespace inspire_rt

@@ -54,7 +54,7 @@ struct Slot {
     const Pair **h_seats = nullptr;
     uint64_t *h_a = nullptr, *h_bp = nullptr;
     int32_t *h_status = nullptr;
-    DevBuf d_ids, d_seeds, d_b, d_seats, d_a, d_bp, scan_part, status;
+    DevBuf d_ids, d_seeds, d_b, d_seats, d_a, d_bp, scan_part, status, group;
     // inputs copied (copy-in stream), kernels done (compute stream), outputs copied (copy-out
     // stream): the last one completes the slot
     hipEvent_t in_ev = nullptr, kern_ev = nullptr, done_ev = nullptr;
@@ -117,6 +117,8 @@ int alloc_slot(inspire_server *sv, Slot &s, char *err, size_t err_cap) {
     HIP_TRY(s.d_bp.alloc(c * g * 8));
     HIP_TRY(s.scan_part.alloc(c * sv->ctx->ks() * g * 4 * 8));
     HIP_TRY(s.status.alloc(c * 4));
+    const size_t gc = c < (size_t)kGroupMax ? c : (size_t)kGroupMax;
+    HIP_TRY(s.group.alloc((2 * gc + 2) * 4));
     HIP_TRY(hipEventCreateWithFlags(&s.in_ev, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&s.kern_ev, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&s.done_ev, hipEventDisableTiming));
@@ -167,6 +169,13 @@ int inspire_server::launch(Slot &s, char *err, size_t err_cap) {
     bd.b_out = s.d_bp.as<uint64_t>();
     bd.status = s.status.as<int32_t>();
     bd.switch_modulus = s.reqs.front().switch_modulus;
+    const size_t gc = s.cap < (size_t)kGroupMax ? s.cap : (size_t)kGroupMax;
+    if (q <= gc) {  // shard grouping of the launch (see BatchDev::order)
+        uint32_t *g = s.group.as<uint32_t>();
+        bd.order = g;
+        bd.first = g + gc;
+        bd.ngroups = g + 2 * gc + 1;
+    }
     HIP_TRY(launch_respond(dev, bd, stream));
     HIP_TRY(ctx->fence.note(stream));
     HIP_TRY(hipEventRecord(s.kern_ev, stream));

@@ -88,7 +88,7 @@ def lib():
         L.inspire_respond_batch_device_noalloc.argtypes = [vp, vp, vp, vp, vp, sz, u64, vp, vp, vp, vp, cp, sz]
         L.inspire_respond_batch_device_timed.argtypes = [vp, vp, vp, vp, vp, vp, sz, u64, vp, vp, vp, vp, cp, sz]
         L.inspire_workspace_trace.argtypes = [vp, sz, cp, sz]
-        L.inspire_workspace_kernel_times.argtypes = [vp, vp, vp, sz, vp, cp, sz]
+        L.inspire_workspace_kernel_times.argtypes = [vp, vp, vp, vp, sz, vp, cp, sz]
         L.inspire_server_create.argtypes = [vp, sz, sz, pp, cp, sz]
         L.inspire_server_destroy.argtypes = [vp]
         L.inspire_server_register_seat.argtypes = [vp, vp, u32, u32, ctypes.POINTER(u64), cp, sz]
@@ -202,13 +202,14 @@ class Workspace:
         self._traced = launches
 
     def kernel_times(self):
-        """(scan_ms, chain_ms) arrays of the traced launches; waits for the last one."""
+        """(group_ms, scan_ms, chain_ms) arrays of the traced launches; waits for them."""
         cap = getattr(self, "_traced", 0)
-        scan, chain = np.zeros(cap, np.float32), np.zeros(cap, np.float32)
+        group, scan, chain = np.zeros(cap, np.float32), np.zeros(cap, np.float32), np.zeros(cap, np.float32)
         n = ctypes.c_size_t()
-        _call(lib().inspire_workspace_kernel_times, self._h, _ptr(scan), _ptr(chain), cap, ctypes.byref(n))
+        _call(lib().inspire_workspace_kernel_times, self._h, _ptr(group), _ptr(scan), _ptr(chain), cap,
+              ctypes.byref(n))
         self._traced = 0
-        return scan[:n.value], chain[:n.value]
+        return group[:n.value], scan[:n.value], chain[:n.value]
 
     def close(self):
         if self._h:

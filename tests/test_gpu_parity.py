@@ -200,8 +200,9 @@ def test_kernel_trace_keeps_results(gpu):
                 ctx.respond_batch_device(seat, ws, ids.data_ptr(), seeds.data_ptr(), b.data_ptr(), L,
                                          a_out.data_ptr(), b_out.data_ptr())
             if traced:
-                scan, chain = ws.kernel_times()
-                assert len(scan) == len(chain) == 2 and (scan > 0).all() and (chain > 0).all()
+                group, scan, chain = ws.kernel_times()
+                assert len(group) == len(scan) == len(chain) == 2
+                assert (group > 0).all() and (scan > 0).all() and (chain > 0).all()
             torch.cuda.synchronize()
             assert (ws.status(L) == 0).all()
             outs.append((a_out.cpu(), b_out.cpu()))
