@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "xcd.hpp"
+
 namespace inspire {
 
 // A fixed multiplicand w < q with its Shoup companion wp = floor(w * 2^64 / q).
@@ -86,17 +88,6 @@ struct BatchDev {
 // Largest launch the grouping kernel sorts (one workgroup, keys in LDS); larger launches scan
 // query by query.
 constexpr int kGroupMax = 4096;
-// XCDs of an MI355X: consecutive workgroups of a launch are dispatched to them in turn.
-constexpr int kXcds = 8;
-
-// Position in the shard-sorted order of the query that workgroup b of `count` takes: XCD x
-// (= b mod 8) gets the x-th contiguous slice of the order, as many positions as it gets
-// workgroups, in dispatch order (a bijection of [0, count)).
-__host__ __device__ inline int xcd_slot(int b, int count) {
-    const int x = b % kXcds, j = b / kXcds, m = count / kXcds, r = count % kXcds;
-    return x * m + (x < r ? x : r) + j;
-}
-
 // Whether a launch is grouped by shard: it carries the group buffers, fits the sort, and its
 // shards are wide enough for the grouped scan's lane layout. The scan and packing kernels both
 // decide by this.

@@ -336,8 +336,9 @@ __global__ __launch_bounds__((1 << LOGN) / 8, 5) void k_sm_respond(SmCtxDev c, S
     __shared__ __align__(16) uint32_t L[3 * NW];
     __shared__ int rejected;
     uint32_t *X = L, *PV = L, *PQ = L + NW, *PP = L + 2 * NW;
-    const int qi = blockIdx.x, t = threadIdx.x;
-    const SmJob job = bt.jobs[qi];
+    const int t = threadIdx.x;
+    const SmJob job = bt.jobs[blockIdx.x];
+    const int qi = job.qi;
     const uint32_t q = c.q, p = c.p;
 
     // --- a = expand_uniform(ChaCha20(seed), q) (server.rs:408), one block of 8 draws per thread

@@ -34,11 +34,13 @@ struct SmCtxDev {
     const Pair32 *twf_q, *twi_q, *twf_p, *twi_p;
 };
 
-// One query's resident inputs.
+// One query's resident inputs. Workgroup b answers jobs[b], whose query sits at position qi of
+// the batch (seeds, bodies, outputs); the host lays the jobs out by sm_order_jobs.
 struct SmJob {
     const uint32_t *step1;  // gamma x n: (n gamma)^-1 R NTT_q(P_i), canonical
     const uint8_t *cells;   // n x cell_bytes, rows past the shard's cells zero
     const uint32_t *seat;   // 4 x n Montgomery operands: n w_q, key_q (mod q), w_p / n, key_p / n (mod p)
+    int qi;
 };
 
 struct SmBatchDev {

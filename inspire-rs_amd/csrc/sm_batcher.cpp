@@ -124,11 +124,12 @@ int inspire_sm_batcher::launch(Slot &s, char *err, size_t err_cap) {
         const size_t c = r.tables.size();
         for (size_t k = 0; k < c; k++)
             s.h_jobs[q + k] = SmJob{r.tables[k]->step1.as<uint32_t>(), r.tables[k]->cells.as<uint8_t>(),
-                                    r.seat->words.as<uint32_t>()};
+                                    r.seat->words.as<uint32_t>(), (int)(q + k)};
         memcpy(s.h_seeds + q * 32, r.seeds.data(), c * 32);
         memcpy(s.h_bodies + q * n, r.bodies.data(), c * n * 4);
         q += c;
     }
+    sm_order_jobs(s.h_jobs, q);
     // the slot's previous launch has completed (its done_ev), so its buffers are free
     SM_TRY(hipMemcpyAsync(s.d_jobs.p, s.h_jobs, q * sizeof(SmJob), hipMemcpyHostToDevice, copy_in));
     SM_TRY(hipMemcpyAsync(s.d_seeds.p, s.h_seeds, q * 32, hipMemcpyHostToDevice, copy_in));

@@ -313,8 +313,9 @@ int check_handles(inspire_sm_server *s, size_t count, const inspire_sm_table *co
             sm_err(err, err_cap, "table or key seat belongs to another parameter set");
             return INSPIRE_ERR_INVALID;
         }
-        jobs[k] = SmJob{t->step1.as<uint32_t>(), t->cells.as<uint8_t>(), st->words.as<uint32_t>()};
+        jobs[k] = SmJob{t->step1.as<uint32_t>(), t->cells.as<uint8_t>(), st->words.as<uint32_t>(), (int)k};
     }
+    sm_order_jobs(jobs.data(), count);
     return INSPIRE_OK;
 }
 

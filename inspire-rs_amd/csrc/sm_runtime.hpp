@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <mutex>
@@ -12,6 +13,7 @@
 #include "fence.hpp"
 #include "sm.hpp"
 #include "trace.hpp"
+#include "xcd.hpp"
 
 namespace inspire_sm {
 
@@ -43,6 +45,21 @@ struct Buf {
     }
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
+
+// Lays out a launch's jobs (built in batch order, qi = position) so that the queries of one
+// seat, then of one table, run next to each other on one XCD: sorted by (seat, table, qi), and
+// workgroup b takes sorted position xcd_slot(b). The chain re-reads the seat's four rows every
+// step; with the queries of a seat together on an XCD those reads come from that XCD's L2.
+inline void sm_order_jobs(inspire::SmJob *jobs, size_t count) {
+    if (count < 2) return;
+    std::vector<inspire::SmJob> sorted(jobs, jobs + count);
+    std::sort(sorted.begin(), sorted.end(), [](const inspire::SmJob &a, const inspire::SmJob &b) {
+        if (a.seat != b.seat) return a.seat < b.seat;
+        if (a.step1 != b.step1) return a.step1 < b.step1;
+        return a.qi < b.qi;
+    });
+    for (size_t b = 0; b < count; b++) jobs[b] = sorted[inspire::xcd_slot((int)b, (int)count)];
+}
 
 }  // namespace inspire_sm
 
