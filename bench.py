@@ -218,9 +218,11 @@ def served(g, seconds, rate_qps, seed=7, capacity_queries=0, queries_per_cu=2):
     batches of 1-32 queries at uniform indices answered by the continuous-batching
     server. Inputs come from host memory (PCIe-inclusive); latency is submit to result.
 
-    Two runs over the same server: arrivals paced at rate_qps (the latency operating
-    point), then, when capacity_queries > 0, that many queries submitted at once as a
-    backlog, whose drain rate is the server's capacity."""
+    Three runs over the same server: arrivals paced at rate_qps (the latency operating
+    point); the same pacing with about 10% of requests opening a new session first (the
+    key is seated by register_seat while launches run, and the request's latency counts
+    the registration); then, when capacity_queries > 0, that many queries submitted at
+    once as a backlog, whose drain rate is the server's capacity."""
     import threading
 
     rng = np.random.default_rng(seed)
@@ -237,7 +239,10 @@ def served(g, seconds, rate_qps, seed=7, capacity_queries=0, queries_per_cu=2):
     # query bodies come from a pool: their values do not change the work
     pool = rng.integers(0, g.Q, (64, 32, N), dtype=np.uint64)
 
-    def schedule(total_s=None, total_q=None):
+    # key rows of the sessions opened during the registering run (seating copies them)
+    new_keys = rng.integers(0, g.Q, (8, 2, N), dtype=np.uint64)
+
+    def schedule(total_s=None, total_q=None, new_frac=0.0):
         reqs, t, q = [], 0.0, 0
         while (total_s is not None and t < total_s) or (total_q is not None and q < total_q):
             count = int(rng.integers(1, 33))
@@ -245,11 +250,12 @@ def served(g, seconds, rate_qps, seed=7, capacity_queries=0, queries_per_cu=2):
             q += count
             reqs.append((t, int(rng.integers(0, len(handles))), rng.integers(0, shards, count).astype(np.uint32),
                          [rng.integers(0, 256, 32, dtype=np.uint8).tobytes() for _ in range(count)],
-                         pool[int(rng.integers(0, len(pool)))][:count]))
+                         pool[int(rng.integers(0, len(pool)))][:count], bool(rng.random() < new_frac)))
         return reqs
 
     def run(reqs, paced):
         lats, done_q = [], [0]
+        new_lats, reg_ms = [], []
         lock = threading.Lock()
         tickets = []
 
@@ -265,9 +271,12 @@ def served(g, seconds, rate_qps, seed=7, capacity_queries=0, queries_per_cu=2):
                 if tk is None:
                     time.sleep(0.0005)
                     continue
+                tk, reg = tk
                 a, b, lat = server.wait(tk)
                 with lock:
-                    lats.append(lat)
+                    lats.append(lat + reg)
+                    if reg > 0:
+                        new_lats.append(lat + reg)
                     done_q[0] += a.shape[0]
 
         waiter.stop = False
@@ -276,24 +285,48 @@ def served(g, seconds, rate_qps, seed=7, capacity_queries=0, queries_per_cu=2):
             th.start()
         launches0 = server.stats()["launches"]
         start = time.perf_counter()
-        for (at, h, ids, seeds, bs) in reqs:
+        for (at, h, ids, seeds, bs, new) in reqs:
             if paced:
                 delay = start + at - time.perf_counter()
                 if delay > 0:
                     time.sleep(delay)
-            tk = server.submit(handles[h], ids, seeds, bs, switch_modulus=Q36)
+            reg = 0.0
+            if new:
+                # a new session: its key is seated before its first request is submitted
+                t_reg = time.perf_counter()
+                handle = server.register_seat(new_keys[len(reg_ms) % len(new_keys)])
+                reg = (time.perf_counter() - t_reg) * 1e3
+                reg_ms.append(reg)
+            else:
+                handle = handles[h]
+            tk = server.submit(handle, ids, seeds, bs, switch_modulus=Q36)
             with lock:
-                tickets.append(tk)
+                tickets.append((tk, reg))
         waiter.stop = True
         for th in ws_threads:
             th.join()
         elapsed = time.perf_counter() - start
+        run.new_lats, run.reg_ms = np.array(new_lats), np.array(reg_ms)
         return done_q[0], elapsed, np.array(lats), server.stats()["launches"] - launches0
+
+    def pct(x):
+        return {"p50": round(float(np.percentile(x, 50)), 3), "p99": round(float(np.percentile(x, 99)), 3)}
 
     reqs = schedule(total_s=seconds)
     # warm the server
     server.wait(server.submit(handles[0], reqs[0][2], reqs[0][3], reqs[0][4], switch_modulus=Q36))
     done, elapsed, lats, launches = run(reqs, paced=True)
+    seats_before = server.stats()["seats"]
+    rreqs = schedule(total_s=seconds, new_frac=0.1)
+    rdone, relapsed, rlats, rlaunches = run(rreqs, paced=True)
+    registering = {"value": round(rdone / relapsed, 1), "unit": "queries/s", "requests": len(rlats),
+                   "new_sessions": len(run.reg_ms), "latency_ms": pct(rlats),
+                   "new_session_latency_ms": pct(run.new_lats) if len(run.new_lats) else None,
+                   "register_ms": pct(run.reg_ms) if len(run.reg_ms) else None,
+                   "seats": {"before": seats_before, "after": server.stats()["seats"]},
+                   "workload": "the paced mix with about 10% of requests opening a new session first: its key "
+                               "is seated (register_seat) while launches run, and the request's latency "
+                               "includes the registration"}
     capacity = None
     if capacity_queries > 0:
         creqs = schedule(total_q=capacity_queries)
@@ -306,11 +339,17 @@ def served(g, seconds, rate_qps, seed=7, capacity_queries=0, queries_per_cu=2):
     server.close()
     ctx.close()
     return {"offered_qps": round(rate_qps, 1), "value": round(done / elapsed, 1), "unit": "queries/s",
-            "latency_ms": {"p50": round(float(np.percentile(lats, 50)), 3),
-                           "p99": round(float(np.percentile(lats, 99)), 3)},
+            "latency_ms": pct(lats),
             "requests": len(lats), "launches": launches,
-            "seats_resident_bytes": stats["resident_bytes"], "store_build_s": round(build, 2),
+            "seats": stats["seats"], "seats_resident_bytes": stats["resident_bytes"],
+            "seats_payload_bytes_reference_accounting": stats["payload_bytes"],
+            "store_build_s": round(build, 2),
+            "registering": registering,
             "capacity": capacity,
+            "parity": "approximate-gadget response bytes: no reference vector exists for them; they rest on the "
+                      "rounding contract of the device digit split, the mask-stream separation, approximate != "
+                      "exact and store == per-query on the GPU, decode correctness of device responses, and "
+                      "byte equality with the oracle's restatement",
             "workload": "d2048_approximate_gadget, 2^20 x 512 B (512 shards), 1024 seated keys, Poisson arrivals "
                         "of 1-32-query batches at 70% of the batched rate, host inputs (PCIe-inclusive), "
                         "continuous batching <= 512 queries per launch, 36-bit switch"}

@@ -240,8 +240,11 @@ int inspire_server_submit(inspire_server *server, uint64_t handle, const uint32_
  * latency_ms receives submit-to-completion time. */
 int inspire_server_wait(inspire_server *server, uint64_t ticket, uint64_t *a_out, uint64_t *b_prefix_out,
                         double *latency_ms, char *err, size_t err_cap);
+/* Launches and queries answered so far; resident_bytes = device bytes of the seated keys (each
+ * key row word with its Shoup companion: 2 x rows x ring_dim x 8 bytes per seat, twice the
+ * reference's PackingKeySeat::payload_bytes, src/pir/session.rs:630-636); seats = keys seated. */
 int inspire_server_stats(inspire_server *server, uint64_t *launches, uint64_t *queries,
-                         uint64_t *resident_bytes);
+                         uint64_t *resident_bytes, uint64_t *seats);
 
 /* ------------------------------------------------------------------------------
  * Special-modulus set (src/special_modulus): q < 2^29 and a special prime p < 2^31 on

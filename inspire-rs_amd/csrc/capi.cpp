@@ -137,6 +137,68 @@ int check_batch(inspire_ctx *ctx, const inspire_seat *seat, const uint32_t *ids,
 
 }  // namespace
 
+int inspire_rt::seat_create_enqueue(inspire_ctx *ctx, const uint64_t *y_body, uint32_t rows, uint32_t num_to_pack,
+                                     inspire_seat **out, char *err, size_t err_cap) {
+    if (!ctx || !y_body || !out) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    const size_t n = ctx->n;  // the context's shape never changes: checked without its lock
+    if (num_to_pack != (uint32_t)ctx->gamma) {
+        set_err(err, err_cap,
+                "packing key was built for gamma = %u but the server packs gamma = %d; the client must re-derive its "
+                "packing keys against the current CRS",
+                num_to_pack, ctx->gamma);
+        return INSPIRE_ERR_INVALID;
+    }
+    if (rows != (uint32_t)ctx->rows) {
+        set_err(err, err_cap,
+                "packing-key seat is for another ring or gadget: it holds %u key rows on ring_dim %zu, the server packs "
+                "%d rows on ring_dim %zu. The client must re-derive its packing keys against the current CRS.",
+                rows, n, ctx->rows, n);
+        return INSPIRE_ERR_INVALID;
+    }
+    for (size_t i = 0; i < rows * n; i++)
+        if (y_body[i] >= kQ) {
+            set_err(err, err_cap, "Poly decode refused: limb 0 coefficient %zu is %llu, not canonical for modulus %llu",
+                    i % n, (unsigned long long)y_body[i], (unsigned long long)kQ);
+            return INSPIRE_ERR_INVALID;
+        }
+    inspire_seat *seat = new (std::nothrow) inspire_seat();
+    if (!seat) { set_err(err, err_cap, "out of host memory"); return INSPIRE_ERR_DEVICE; }
+    seat->ctx = ctx;
+    seat->rows = rows;
+    seat->num_to_pack = num_to_pack;
+    // Enqueued on the context's stream under the lock: a registration while launches run
+    // allocates and frees nothing that waits for the device (its memory comes from the seat
+    // pool, the rows go through the context's staging buffer, which the next registration
+    // overwrites only behind these kernels on the same stream), and nothing waits here.
+    hipError_t e = hipEventCreateWithFlags(&seat->ready, hipEventDisableTiming);
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->seat_pool.block = (size_t)rows * n * sizeof(Pair);
+        void *mem = nullptr;
+        if (e == hipSuccess) e = ctx->seat_pool.take(&mem);
+        if (e == hipSuccess) {
+            seat->yprime.p = mem;
+            seat->yprime.bytes = ctx->seat_pool.block;
+        }
+        if (e == hipSuccess && !ctx->seat_staging.p) e = ctx->seat_staging.alloc((size_t)rows * n * 8);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ctx->seat_staging.p, y_body, rows * n * 8, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess)
+            e = launch_seat(ctx->logn, (int)rows, ctx->seat_staging.as<uint64_t>(), ctx->psi.as<uint64_t>(),
+                            ctx->inv_n, seat->yprime.as<Pair>(), ctx->stream);
+        if (e == hipSuccess && rows == 3)
+            e = launch_fold_rows(seat->yprime.as<Pair>(), (int)n, ctx->kx.w, ctx->k0.w, ctx->k1.w, ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(seat->ready, ctx->stream);
+    }
+    if (e != hipSuccess) {
+        set_err(err, err_cap, "HIP failure seating the packing key: %s", hipGetErrorString(e));
+        delete seat;
+        return INSPIRE_ERR_DEVICE;
+    }
+    *out = seat;
+    return INSPIRE_OK;
+}
+
 extern "C" {
 
 int inspire_ctx_create(const inspire_params *params, const uint8_t w_seed[32], uint32_t num_columns,
@@ -387,49 +449,16 @@ void inspire_shard_destroy(inspire_shard *shard) {
 
 int inspire_seat_create(inspire_ctx *ctx, const uint64_t *y_body, uint32_t rows, uint32_t num_to_pack,
                         inspire_seat **out, char *err, size_t err_cap) {
-    if (!ctx || !y_body || !out) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    const size_t n = ctx->n;
-    if (num_to_pack != (uint32_t)ctx->gamma) {
-        set_err(err, err_cap,
-                "packing key was built for gamma = %u but the server packs gamma = %d; the client must re-derive its "
-                "packing keys against the current CRS",
-                num_to_pack, ctx->gamma);
-        return INSPIRE_ERR_INVALID;
-    }
-    if (rows != (uint32_t)ctx->rows) {
-        set_err(err, err_cap,
-                "packing-key seat is for another ring or gadget: it holds %u key rows on ring_dim %zu, the server packs "
-                "%d rows on ring_dim %zu. The client must re-derive its packing keys against the current CRS.",
-                rows, n, ctx->rows, n);
-        return INSPIRE_ERR_INVALID;
-    }
-    for (size_t i = 0; i < rows * n; i++)
-        if (y_body[i] >= kQ) {
-            set_err(err, err_cap, "Poly decode refused: limb 0 coefficient %zu is %llu, not canonical for modulus %llu",
-                    i % n, (unsigned long long)y_body[i], (unsigned long long)kQ);
-            return INSPIRE_ERR_INVALID;
-        }
-    inspire_seat *seat = new (std::nothrow) inspire_seat();
-    if (!seat) { set_err(err, err_cap, "out of host memory"); return INSPIRE_ERR_DEVICE; }
-    seat->ctx = ctx;
-    seat->rows = rows;
-    seat->num_to_pack = num_to_pack;
-    DevBuf tmp;
-    hipError_t e = seat->yprime.alloc(rows * n * sizeof(Pair));
-    if (e == hipSuccess) e = tmp.alloc(rows * n * 8);
-    if (e == hipSuccess) e = hipMemcpy(tmp.p, y_body, rows * n * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = launch_seat(ctx->logn, (int)rows, tmp.as<uint64_t>(), ctx->psi.as<uint64_t>(), ctx->inv_n,
-                        seat->yprime.as<Pair>(), ctx->stream);
-    if (e == hipSuccess && rows == 3)
-        e = launch_fold_rows(seat->yprime.as<Pair>(), (int)n, ctx->kx.w, ctx->k0.w, ctx->k1.w, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    inspire_seat *seat = nullptr;
+    int rc = seat_create_enqueue(ctx, y_body, rows, num_to_pack, &seat, err, err_cap);
+    if (rc) return rc;
+    hipError_t e = hipEventSynchronize(seat->ready);
     if (e != hipSuccess) {
         set_err(err, err_cap, "HIP failure seating the packing key: %s", hipGetErrorString(e));
-        delete seat;
+        inspire_seat_destroy(seat);
         return INSPIRE_ERR_DEVICE;
     }
+    seat->settled = true;
     *out = seat;
     return INSPIRE_OK;
 }
@@ -444,7 +473,14 @@ static std::vector<hipEvent_t> launches_so_far(inspire_ctx *ctx) {
 
 void inspire_seat_destroy(inspire_seat *seat) {
     if (!seat) return;
-    (void)wait_for(launches_so_far(seat->ctx));
+    inspire_ctx *ctx = seat->ctx;
+    (void)wait_for(launches_so_far(ctx));
+    {
+        // no launch enqueued so far reads the seat any more: its memory goes back to the pool
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->seat_pool.give(seat->yprime.p);
+        seat->yprime.p = nullptr;
+    }
     delete seat;
 }
 

@@ -61,6 +61,29 @@ struct LaunchFence {
     }
 };
 
+// Device blocks of one size kept for reuse. hipFree waits for the whole device, so the seats
+// that sessions open and close while launches run take their memory from here and give it back
+// (after the fence wait of their destroy) instead of allocating and freeing it. Caller holds
+// the owner's lock.
+struct BlockPool {
+    size_t block = 0;                 // bytes of every block
+    std::vector<void *> free_blocks;  // no launch reads these any more
+    ~BlockPool() {
+        for (void *p : free_blocks) (void)hipFree(p);
+    }
+    hipError_t take(void **out) {
+        if (!free_blocks.empty()) {
+            *out = free_blocks.back();
+            free_blocks.pop_back();
+            return hipSuccess;
+        }
+        return hipMalloc(out, block);
+    }
+    void give(void *p) {
+        if (p) free_blocks.push_back(p);
+    }
+};
+
 inline hipError_t wait_for(const std::vector<hipEvent_t> &events) {
     hipError_t first = hipSuccess;
     for (hipEvent_t e : events) {

@@ -112,6 +112,13 @@ struct inspire_seat {
     inspire_ctx *ctx = nullptr;
     uint32_t rows = 0, num_to_pack = 0;
     DevBuf yprime;
+    // recorded on the context's stream behind the kernels that fill yprime; a launch on another
+    // stream waits for it until it has been seen complete (settled)
+    hipEvent_t ready = nullptr;
+    bool settled = false;
+    ~inspire_seat() {
+        if (ready) (void)hipEventDestroy(ready);
+    }
 };
 
 struct inspire_workspace {
@@ -145,6 +152,10 @@ struct inspire_ctx {
     // every launch that reads the shard table, a shard, a seat or a workspace records an event
     // here under mu, so destroying one of them waits for those launches only
     LaunchFence fence;
+    // seats' device memory (rows x n Pairs each), reused across registrations; the host rows of
+    // a seat being created are staged in seat_staging (rows x n words), ordered on stream
+    BlockPool seat_pool;
+    DevBuf seat_staging;
     ~inspire_ctx() {
         if (stream) {
             (void)hipStreamSynchronize(stream);
@@ -194,6 +205,11 @@ inline int sync_table(inspire_ctx *ctx, char *err, size_t err_cap) {
 // a shard from its columns on every query, so an edit of the shard drops this store (the next
 // respond builds it again from the edited columns) instead of failing its tag check.
 int build_store(inspire_ctx *ctx, inspire_shard *sh, bool lazy, char *err, size_t err_cap);
+
+// inspire_seat_create without the wait: the seat's kernels are enqueued on the context's stream
+// and seat->ready recorded behind them (capi.cpp).
+int seat_create_enqueue(inspire_ctx *ctx, const uint64_t *y_body, uint32_t rows, uint32_t num_to_pack,
+                        inspire_seat **out, char *err, size_t err_cap);
 
 inline int ws_reserve(inspire_workspace *ws, size_t count, char *err, size_t err_cap) {
     inspire_ctx *ctx = ws->ctx;

@@ -155,6 +155,15 @@ int inspire_server::launch(Slot &s, char *err, size_t err_cap) {
     HIP_TRY(hipMemcpyAsync(s.d_seats.p, s.h_seats, q * sizeof(void *), hipMemcpyHostToDevice, copy_in));
     HIP_TRY(hipEventRecord(s.in_ev, copy_in));
     HIP_TRY(hipStreamWaitEvent(stream, s.in_ev, 0));
+    // seats registered since their kernels were last seen complete: this launch waits for them
+    for (const Request &r : s.reqs) {
+        inspire_seat *st = r.seat.get();
+        if (st->settled) continue;
+        if (hipEventQuery(st->ready) == hipSuccess)
+            st->settled = true;
+        else
+            HIP_TRY(hipStreamWaitEvent(stream, st->ready, 0));
+    }
     BatchDev bd;
     bd.count = (int)q;
     bd.shard_ids = s.d_ids.as<uint32_t>();
@@ -354,8 +363,10 @@ int inspire_server_register_seat(inspire_server *sv, const uint64_t *y_body, uin
             return INSPIRE_ERR_INVALID;
         }
     }
+    // the seat's kernels are enqueued on the context's stream and not waited for: the launch
+    // that first carries one of its requests waits for them on the device (launch())
     inspire_seat *seat = nullptr;
-    int rc = inspire_seat_create(sv->ctx, y_body, rows, num_to_pack, &seat, err, err_cap);
+    int rc = seat_create_enqueue(sv->ctx, y_body, rows, num_to_pack, &seat, err, err_cap);
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(sv->mu);
     const uint64_t h = sv->next_handle++;
@@ -367,6 +378,10 @@ int inspire_server_register_seat(inspire_server *sv, const uint64_t *y_body, uin
 
 int inspire_server_unregister_seat(inspire_server *sv, uint64_t handle, char *err, size_t err_cap) {
     if (!sv) { set_err(err, err_cap, "null server"); return INSPIRE_ERR_INVALID; }
+    // queued requests keep the seat alive until they complete; otherwise it is destroyed when
+    // `dropped` goes, after the lock (the destroy waits for the launches enqueued so far, and
+    // the worker must not wait behind it)
+    std::shared_ptr<inspire_seat> dropped;
     std::lock_guard<std::mutex> lk(sv->mu);
     auto it = sv->seats.find(handle);
     if (it == sv->seats.end()) {
@@ -375,7 +390,8 @@ int inspire_server_unregister_seat(inspire_server *sv, uint64_t handle, char *er
         return INSPIRE_ERR_INVALID;
     }
     sv->resident -= (size_t)it->second->rows * sv->ctx->n * sizeof(Pair);
-    sv->seats.erase(it);  // queued requests keep the seat alive until they complete
+    dropped = std::move(it->second);
+    sv->seats.erase(it);
     return INSPIRE_OK;
 }
 
@@ -481,12 +497,14 @@ int inspire_server_wait(inspire_server *sv, uint64_t ticket, uint64_t *a_out, ui
     return INSPIRE_OK;
 }
 
-int inspire_server_stats(inspire_server *sv, uint64_t *launches, uint64_t *queries, uint64_t *resident_bytes) {
+int inspire_server_stats(inspire_server *sv, uint64_t *launches, uint64_t *queries, uint64_t *resident_bytes,
+                         uint64_t *seats) {
     if (!sv) return INSPIRE_ERR_INVALID;
     std::lock_guard<std::mutex> lk(sv->mu);
     if (launches) *launches = sv->launches.load();
     if (queries) *queries = sv->queries.load();
     if (resident_bytes) *resident_bytes = sv->resident;
+    if (seats) *seats = sv->seats.size();
     return INSPIRE_OK;
 }
 

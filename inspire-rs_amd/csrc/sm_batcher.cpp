@@ -136,6 +136,15 @@ int inspire_sm_batcher::launch(Slot &s, char *err, size_t err_cap) {
     SM_TRY(hipMemcpyAsync(s.d_bodies.p, s.h_bodies, q * n * 4, hipMemcpyHostToDevice, copy_in));
     SM_TRY(hipEventRecord(s.in_ev, copy_in));
     SM_TRY(hipStreamWaitEvent(stream, s.in_ev, 0));
+    // seats registered since their kernel was last seen complete: this launch waits for them
+    for (const Request &r : s.reqs) {
+        inspire_sm_seat *st = r.seat.get();
+        if (st->settled) continue;
+        if (hipEventQuery(st->ready) == hipSuccess)
+            st->settled = true;
+        else
+            SM_TRY(hipStreamWaitEvent(stream, st->ready, 0));
+    }
     SmBatchDev bd{(int)q, s.d_jobs.as<SmJob>(), s.d_seeds.as<uint8_t>(), s.d_bodies.as<uint32_t>(),
                   s.d_mask.as<uint32_t>(), s.d_body.as<uint32_t>()};
     {
@@ -302,8 +311,10 @@ int inspire_sm_batcher_register_key(inspire_sm_batcher *b, const uint8_t mask_se
             return INSPIRE_ERR_INVALID;
         }
     }
+    // the seat's kernel is enqueued and not waited for: the launch that first carries one of its
+    // requests waits for it on the device (launch())
     inspire_sm_seat *seat = nullptr;
-    int rc = inspire_sm_seat_create(b->srv, mask_seed, key_q, key_p, &seat, err, err_cap);
+    int rc = seat_create_enqueue(b->srv, mask_seed, key_q, key_p, &seat, err, err_cap);
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(b->mu);
     const uint64_t h = b->next_handle++;
@@ -315,6 +326,9 @@ int inspire_sm_batcher_register_key(inspire_sm_batcher *b, const uint8_t mask_se
 
 int inspire_sm_batcher_unregister_key(inspire_sm_batcher *b, uint64_t handle, char *err, size_t err_cap) {
     if (!b) { sm_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    // queued requests keep the seat alive until they complete; otherwise it is destroyed when
+    // `dropped` goes, after the lock (the destroy waits for the launches enqueued so far)
+    std::shared_ptr<inspire_sm_seat> dropped;
     std::lock_guard<std::mutex> lk(b->mu);
     auto it = b->seats.find(handle);
     if (it == b->seats.end()) {
@@ -322,7 +336,8 @@ int inspire_sm_batcher_unregister_key(inspire_sm_batcher *b, uint64_t handle, ch
         return INSPIRE_ERR_INVALID;
     }
     b->resident -= 4 * b->n * 4;
-    b->seats.erase(it);  // queued requests keep the seat alive until they complete
+    dropped = std::move(it->second);
+    b->seats.erase(it);
     return INSPIRE_OK;
 }
 

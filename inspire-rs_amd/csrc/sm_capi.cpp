@@ -63,6 +63,49 @@ uint64_t first_root(uint64_t m, uint64_t n) {
 
 }  // namespace
 
+int inspire_sm::seat_create_enqueue(inspire_sm_server *s, const uint8_t mask_seed[32], const uint32_t *key_q,
+                                    const uint32_t *key_p, inspire_sm_seat **out, char *err, size_t err_cap) {
+    if (!s || !out || !mask_seed || !key_q || !key_p) { sm_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    const size_t n = s->params.ring_dim;
+    for (size_t k = 0; k < n; k++)
+        if (key_q[k] >= s->params.q || key_p[k] >= s->params.special_prime) {
+            sm_err(err, err_cap, "packing key coefficient is not a residue");
+            return INSPIRE_ERR_INVALID;
+        }
+    inspire_sm_seat *st = new (std::nothrow) inspire_sm_seat();
+    if (!st) { sm_err(err, err_cap, "out of host memory"); return INSPIRE_ERR_DEVICE; }
+    st->srv = s;
+    // enqueued on the server's stream under its lock, with memory from the seat pool and the
+    // inputs staged in seat_in (the next seat overwrites them only behind this kernel on the
+    // same stream): nothing here waits
+    hipError_t e = hipEventCreateWithFlags(&st->ready, hipEventDisableTiming);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->seat_pool.block = 4 * n * 4;
+        void *mem = nullptr;
+        if (e == hipSuccess) e = s->seat_pool.take(&mem);
+        if (e == hipSuccess) {
+            st->words.p = mem;
+            st->words.bytes = s->seat_pool.block;
+        }
+        if (e == hipSuccess && !s->seat_in.p) e = s->seat_in.alloc(2 * n * 4 + 32);
+        uint32_t *kq = s->seat_in.as<uint32_t>(), *kp = kq + n;
+        uint8_t *seed = reinterpret_cast<uint8_t *>(kp + n);
+        if (e == hipSuccess) e = hipMemcpyAsync(kq, key_q, n * 4, hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(kp, key_p, n * 4, hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(seed, mask_seed, 32, hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess) e = sm_launch_seat(s->dev, seed, kq, kp, st->words.as<uint32_t>(), s->stream);
+        if (e == hipSuccess) e = hipEventRecord(st->ready, s->stream);
+    }
+    if (e != hipSuccess) {
+        sm_err(err, err_cap, "HIP failure seating the packing key: %s", hipGetErrorString(e));
+        delete st;
+        return INSPIRE_ERR_DEVICE;
+    }
+    *out = st;
+    return INSPIRE_OK;
+}
+
 extern "C" {
 
 int inspire_sm_params_validate(const inspire_sm_params *p, char *err, size_t err_cap) {
@@ -260,32 +303,16 @@ void inspire_sm_table_destroy(inspire_sm_table *t) {
 
 int inspire_sm_seat_create(inspire_sm_server *s, const uint8_t mask_seed[32], const uint32_t *key_q,
                            const uint32_t *key_p, inspire_sm_seat **out, char *err, size_t err_cap) {
-    if (!s || !out || !mask_seed || !key_q || !key_p) { sm_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
-    const size_t n = s->params.ring_dim;
-    for (size_t k = 0; k < n; k++)
-        if (key_q[k] >= s->params.q || key_p[k] >= s->params.special_prime) {
-            sm_err(err, err_cap, "packing key coefficient is not a residue");
-            return INSPIRE_ERR_INVALID;
-        }
-    inspire_sm_seat *st = new (std::nothrow) inspire_sm_seat();
-    if (!st) { sm_err(err, err_cap, "out of host memory"); return INSPIRE_ERR_DEVICE; }
-    st->srv = s;
-    std::lock_guard<std::mutex> lk(s->mu);
-    Buf in;
-    hipError_t e = st->words.alloc(4 * n * 4);
-    if (e == hipSuccess) e = in.alloc(2 * n * 4 + 32);
-    uint32_t *kq = in.as<uint32_t>(), *kp = kq + n;
-    uint8_t *seed = reinterpret_cast<uint8_t *>(kp + n);
-    if (e == hipSuccess) e = hipMemcpy(kq, key_q, n * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(kp, key_p, n * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(seed, mask_seed, 32, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = sm_launch_seat(s->dev, seed, kq, kp, st->words.as<uint32_t>(), s->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    inspire_sm_seat *st = nullptr;
+    int rc = seat_create_enqueue(s, mask_seed, key_q, key_p, &st, err, err_cap);
+    if (rc) return rc;
+    hipError_t e = hipEventSynchronize(st->ready);
     if (e != hipSuccess) {
         sm_err(err, err_cap, "HIP failure seating the packing key: %s", hipGetErrorString(e));
-        delete st;
+        inspire_sm_seat_destroy(st);
         return INSPIRE_ERR_DEVICE;
     }
+    st->settled = true;
     *out = st;
     return INSPIRE_OK;
 }
@@ -294,7 +321,14 @@ size_t inspire_sm_seat_resident_bytes(const inspire_sm_seat *st) { return st ? s
 
 void inspire_sm_seat_destroy(inspire_sm_seat *st) {
     if (!st) return;
-    (void)wait_for(st->srv->launches_so_far());
+    inspire_sm_server *s = st->srv;
+    (void)wait_for(s->launches_so_far());
+    {
+        // no launch enqueued so far reads the seat any more: its memory goes back to the pool
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->seat_pool.give(st->words.p);
+        st->words.p = nullptr;
+    }
     delete st;
 }
 

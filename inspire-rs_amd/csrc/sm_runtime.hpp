@@ -85,6 +85,10 @@ struct inspire_sm_server {
     // an event behind every launch that reads this server's tables, seats or job list, recorded
     // under mu: freeing one of them waits for those launches only, not the whole device
     inspire::LaunchFence fence;
+    // seats' device memory (4 x n words each), reused across registrations; a seat being
+    // created stages its key limbs and mask seed in seat_in (2 n words + 32 bytes), on stream
+    inspire::BlockPool seat_pool;
+    inspire_sm::Buf seat_in;
     ~inspire_sm_server() {
         if (stream) {
             (void)hipStreamSynchronize(stream);
@@ -107,4 +111,18 @@ struct inspire_sm_table {
 struct inspire_sm_seat {
     inspire_sm_server *srv = nullptr;
     inspire_sm::Buf words;
+    // recorded on the server's stream behind the kernel that fills words; a launch on another
+    // stream waits for it until it has been seen complete (settled)
+    hipEvent_t ready = nullptr;
+    bool settled = false;
+    ~inspire_sm_seat() {
+        if (ready) (void)hipEventDestroy(ready);
+    }
 };
+
+namespace inspire_sm {
+// inspire_sm_seat_create without the wait: the seat's kernel is enqueued on the server's stream
+// and seat->ready recorded behind it (sm_capi.cpp).
+int seat_create_enqueue(inspire_sm_server *s, const uint8_t mask_seed[32], const uint32_t *key_q,
+                        const uint32_t *key_p, inspire_sm_seat **out, char *err, size_t err_cap);
+}  // namespace inspire_sm

@@ -95,7 +95,8 @@ def lib():
         L.inspire_server_unregister_seat.argtypes = [vp, u64, cp, sz]
         L.inspire_server_submit.argtypes = [vp, u64, vp, vp, vp, sz, u64, ctypes.POINTER(u64), cp, sz]
         L.inspire_server_wait.argtypes = [vp, u64, vp, vp, ctypes.POINTER(ctypes.c_double), cp, sz]
-        L.inspire_server_stats.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64)]
+        L.inspire_server_stats.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64),
+                                           ctypes.POINTER(u64)]
         L.inspire_workspace_status.argtypes = [vp, sz, vp, cp, sz]
         L.inspire_response_wire_bytes.argtypes = [vp, u64]
         L.inspire_response_wire_bytes.restype = sz
@@ -424,9 +425,13 @@ class Server:
         return a, b, lat.value
 
     def stats(self):
-        l, q, r = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
-        lib().inspire_server_stats(self._h, ctypes.byref(l), ctypes.byref(q), ctypes.byref(r))
-        return {"launches": l.value, "queries": q.value, "resident_bytes": r.value}
+        """Launches and queries so far, and the seated keys: resident_bytes on the device (each
+        key word with its Shoup companion) and payload_bytes in the reference's accounting
+        (PackingKeySeat::payload_bytes, src/pir/session.rs:630-636: rows x ring_dim x 8 per key)."""
+        l, q, r, k = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        lib().inspire_server_stats(self._h, ctypes.byref(l), ctypes.byref(q), ctypes.byref(r), ctypes.byref(k))
+        return {"launches": l.value, "queries": q.value, "resident_bytes": r.value, "seats": k.value,
+                "payload_bytes": k.value * self.ctx.key_rows * self.ctx.n * 8}
 
     def close(self):
         if self._h:

@@ -4,6 +4,9 @@
   the destroy waits for those launches (their fence events), which are answered normally; a
   launch enqueued afterwards gets status -1 for the destroyed shard and answers the others.
   Seats and workspaces destroyed behind launches likewise wait for them, not for the device.
+* A seat's device memory goes back to the context's seat pool when the seat is destroyed
+  (after the launches enqueued before), and a seat created afterwards takes it: that seat
+  answers under its own key, and the launches behind the destroy under the old one.
 * A shard uploaded without precompute_packing is packed through a store the respond builds on
   the way; after an edit it is packed from the edited columns (as the reference packs a
   store-less shard, src/pir/respond.rs:1022-1041), not refused. A shard whose store came from
@@ -112,3 +115,47 @@ def test_store_less_shard_is_packed_from_its_edited_columns(gpu):
         with pytest.raises(gpu.PirError) as e:
             ctx.respond_seeded_inspiring(y, 1, qseed, b)
         assert e.value.code == gpu.ERR_STORE
+
+
+def test_seat_memory_reused_behind_launches(gpu):
+    import torch
+
+    rng = np.random.default_rng(33)
+    dev = torch.device("cuda:0")
+    gamma = 16
+    w_seed = bytes(range(7, 39))
+    with gpu.Context(gpu.params_d2048(), w_seed, gamma) as ctx:
+        cols = [rng.integers(0, 1 << 16, (gamma, N), dtype=np.uint64) for _ in range(2)]
+        for i, c in enumerate(cols):
+            ctx.upload_columns(i, c).precompute_packing()
+        ya, yb = (rng.integers(0, Q, (3, N), dtype=np.uint64) for _ in range(2))
+        B = 32
+        ids = (np.arange(B) % 2).astype(np.int32)
+        seeds = rng.integers(0, 256, (B, 32), dtype=np.uint8)
+        bs = rng.integers(0, Q, (B, N), dtype=np.uint64)
+        seat_a = ctx.seat(ya)
+        want_a = ctx.respond_seeded_batch(seat_a, ids, [s.tobytes() for s in seeds], bs)
+        ws = gpu.Workspace(ctx, B)
+        a = torch.zeros((B, N), dtype=torch.int64, device=dev)
+        b = torch.zeros((B, gamma), dtype=torch.int64, device=dev)
+        stream = torch.cuda.Stream()
+        torch.cuda.synchronize()
+        print("lifetimes: seat destroyed behind a launch, a new seat takes its memory", flush=True)
+        d_ids, d_seeds = torch.from_numpy(ids).to(dev), torch.from_numpy(seeds).to(dev)
+        d_b = torch.from_numpy(bs.view(np.int64)).to(dev)
+        torch.cuda.synchronize()
+        ctx.respond_batch_device(seat_a, ws, d_ids.data_ptr(), d_seeds.data_ptr(), d_b.data_ptr(), B, a.data_ptr(),
+                                 b.data_ptr(), stream=stream.cuda_stream)
+        seat_a.close()  # waits for the launch; the block goes to the pool
+        seat_b = ctx.seat(yb)
+        stream.synchronize()
+        assert (ws.status(B) == 0).all()
+        assert np.array_equal(a.cpu().numpy().view(np.uint64), want_a[0])
+        assert np.array_equal(b.cpu().numpy().view(np.uint64), want_a[1])
+        got = ctx.respond_seeded_batch(seat_b, ids[:2], [s.tobytes() for s in seeds[:2]], bs[:2])
+        p = oracle.Pack(N, gamma)
+        for k in range(2):
+            wa, wb = p.respond(w_seed, oracle.expand_seed(seeds[k].tobytes(), N), bs[k], cols[ids[k]], yb)
+            assert np.array_equal(got[0][k], wa) and np.array_equal(got[1][k], wb[:gamma]), k
+        seat_b.close()
+        ws.close()

@@ -89,7 +89,11 @@ def test_admission_and_handles(gpu):
         server.register_seat(rng.integers(0, Q, (2, N), dtype=np.uint64))
     server.unregister_seat(hs[0])
     h4 = server.register_seat(rng.integers(0, Q, (2, N), dtype=np.uint64))  # fits again
-    assert server.stats()["resident_bytes"] == 3 * seat_bytes
+    st = server.stats()
+    # device bytes carry each word's Shoup companion; the reference counts 32 KiB per
+    # approximate-gadget seat (2 rows x 2048 x 8, tests/approximate_packing_gadget.rs:519)
+    assert st["resident_bytes"] == 3 * seat_bytes and st["seats"] == 3
+    assert st["payload_bytes"] == 3 * 32_768
     with pytest.raises(gpu.PirError, match="no entry for session_handle"):
         server.submit(hs[0], [0], [bytes(32)], np.zeros((1, N), dtype=np.uint64))
     a, b, _ = server.wait(server.submit(h4, [0], [bytes(32)], np.zeros((1, N), dtype=np.uint64)))
@@ -116,5 +120,44 @@ def test_admission_and_handles(gpu):
     assert "collected by another wait" in refused or "unknown ticket" in refused  # which depends on timing
     with pytest.raises(gpu.PirError, match="unknown ticket"):
         server.wait(ticket)
+    server.close()
+    ctx.close()
+
+
+def test_sessions_opened_and_closed_while_requests_run(gpu):
+    """Seats registered and unregistered between queued requests: an unregistered seat stays
+    alive for its queued requests, its device block returns to the seat pool only after them,
+    and a seat registered afterwards (which may take that block) answers under its own key.
+    Every batch equals the same batch answered alone."""
+    gamma, shards = 32, 4
+    rng, w_seed, ctx, cols = _setup(gpu, True, gamma, shards, seed=11)
+    server = gpu.Server(ctx, max_launch=64)
+    keys = [rng.integers(0, Q, (2, N), dtype=np.uint64) for _ in range(6)]
+    live = {0: server.register_seat(keys[0]), 1: server.register_seat(keys[1])}
+    reqs, tickets = [], []
+
+    def submit(kidx, count):
+        ids = rng.integers(0, shards, count).astype(np.uint32)
+        seeds = [rng.integers(0, 256, 32, dtype=np.uint8).tobytes() for _ in range(count)]
+        bs = rng.integers(0, Q, (count, N), dtype=np.uint64)
+        reqs.append((kidx, ids, seeds, bs))
+        tickets.append(server.submit(live[kidx], ids, seeds, bs, switch_modulus=oracle.Q36))
+
+    print("serving: sessions opened and closed between queued requests", flush=True)
+    for k in range(2, 6):
+        submit(k - 2, 24)
+        submit(k - 1, 7)
+        server.unregister_seat(live.pop(k - 2))  # its queued request keeps it alive
+        live[k] = server.register_seat(keys[k])
+        submit(k, 16)
+    results = [server.wait(t) for t in tickets]
+    st = server.stats()
+    assert st["seats"] == len(live) == 2 and st["resident_bytes"] == 2 * 2 * N * 16
+    seats = [ctx.seat(k) for k in keys]
+    for i, (kidx, ids, seeds, bs) in enumerate(reqs):
+        a_alone, b_alone = ctx.respond_seeded_batch(seats[kidx], ids, seeds, bs, switch_modulus=oracle.Q36)
+        assert np.array_equal(results[i][0], a_alone) and np.array_equal(results[i][1], b_alone), i
+    for s in seats:
+        s.close()
     server.close()
     ctx.close()

@@ -93,3 +93,45 @@ def test_admission_and_handles(gpu):
         assert masks.shape == (1, 2048) and bodies.shape == (1, 492)
         batcher.close()
         table.close()
+
+
+def test_keys_opened_and_closed_while_requests_run(gpu):
+    """Keys registered (their seat kernel not waited for by the registration) and unregistered
+    between queued requests: each request equals the same batch answered alone under a seat of
+    its key, including requests submitted right after their key's registration and requests
+    under a key whose seat memory came back from an unregistered one."""
+    params = gpu.ring2048_cell256()
+    rng = np.random.default_rng(23)
+    cells = [oracle.sm_kat_cells(310 + i, 2048).tobytes() for i in range(2)]
+    with gpu.SmServer(params) as server:
+        tables = [server.table(c) for c in cells]
+        batcher = gpu.SmBatcher(server, max_launch=64)
+        keys = [_key(gpu, rng) for _ in range(6)]
+        live = {0: batcher.register_key(keys[0]), 1: batcher.register_key(keys[1])}
+        reqs, tickets = [], []
+
+        def submit(kidx, count):
+            tids = [int(x) for x in rng.integers(0, len(tables), count)]
+            queries = [gpu.Query(bytes(rng.integers(0, 256, 32, dtype=np.uint8)),
+                                 rng.integers(0, SM_Q, 2048, dtype=np.uint32)) for _ in range(count)]
+            reqs.append((kidx, tids, queries))
+            tickets.append(batcher.submit(live[kidx], [tables[j] for j in tids], queries))
+
+        print("sm serving: keys opened and closed between queued requests", flush=True)
+        for k in range(2, 6):
+            submit(k - 2, 20)
+            submit(k - 1, 5)
+            batcher.unregister_key(live.pop(k - 2))  # its queued request keeps it alive
+            live[k] = batcher.register_key(keys[k])
+            submit(k, 12)
+        results = [batcher.wait(t) for t in tickets]
+        assert batcher.stats()["resident_bytes"] == 2 * 32_768
+        seats = [server.seat(k) for k in keys]
+        for i, (kidx, tids, queries) in enumerate(reqs):
+            masks, bodies = server.respond_batch([tables[j] for j in tids], [seats[kidx]] * len(queries), queries)
+            assert np.array_equal(results[i][0], masks) and np.array_equal(results[i][1], bodies), i
+        for s in seats:
+            s.close()
+        batcher.close()
+        for t in tables:
+            t.close()
