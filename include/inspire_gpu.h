@@ -49,11 +49,19 @@ typedef struct {
     uint32_t approximate_packing_gadget;  /* 1: the key has no row for the lowest digit */
 } inspire_params;
 
-/* PackParams::try_new + OfflinePackingKeys::generate for one CRS and shard width,
- * i.e. ServerInspiringCache::new (src/pir/respond.rs:926-941,
- * src/inspiring/inspiring2.rs:171-286, 537-598). num_columns is gamma. */
+/* PackParams::try_new for one shard width (src/inspiring/inspiring2.rs:171-286): the ring
+ * tables every store and launch reads; num_columns is gamma. With w_seed (the CRS's seed of
+ * the packing mask), also OfflinePackingKeys::generate, i.e. ServerInspiringCache::new
+ * (src/pir/respond.rs:926-941, inspiring2.rs:537-598). With w_seed NULL the context holds no
+ * CRS: it takes shards and builds their packing stores, as ShardData::precompute_packing
+ * (src/pir/setup.rs:335-340) needs only PackParams and the NttContext, and every respond
+ * entry point and inspire_server_create refuse it until inspire_ctx_attach_crs. */
 int inspire_ctx_create(const inspire_params *params, const uint8_t w_seed[32], uint32_t num_columns,
                        inspire_ctx **out, char *err, size_t err_cap);
+/* ServerInspiringCache::new(&crs, &encoded) over a context created without a CRS: derives the
+ * offline packing keys from w_seed. Binding the seed the context already holds is a no-op;
+ * another seed is refused (the context's seats belong to its CRS). */
+int inspire_ctx_attach_crs(inspire_ctx *ctx, const uint8_t w_seed[32], char *err, size_t err_cap);
 void inspire_ctx_destroy(inspire_ctx *ctx);
 /* key rows the seats must carry: packing_gadget_len minus the dropped digits
  * (PackParams::key_rows, src/inspiring/inspiring2.rs:289-291) */

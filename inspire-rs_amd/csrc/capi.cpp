@@ -111,6 +111,10 @@ int register_shard(inspire_ctx *ctx, uint32_t id, const std::vector<uint16_t> &r
 // and shard before packing: respond.rs:409-611).
 int check_batch(inspire_ctx *ctx, const inspire_seat *seat, const uint32_t *ids, size_t count, char *err,
                 size_t err_cap) {
+    if (!ctx->has_crs) {
+        set_err(err, err_cap, kNoCrs);
+        return INSPIRE_ERR_INVALID;
+    }
     if (!seat || seat->ctx != ctx) {
         set_err(err, err_cap, "packing-key seat was built for another context");
         return INSPIRE_ERR_INVALID;
@@ -203,7 +207,7 @@ extern "C" {
 
 int inspire_ctx_create(const inspire_params *params, const uint8_t w_seed[32], uint32_t num_columns,
                        inspire_ctx **out, char *err, size_t err_cap) {
-    if (!params || !w_seed || !out) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    if (!params || !out) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
     *out = nullptr;
     const uint32_t n = params->ring_dim;
     if (n != 256 && n != 2048) {
@@ -237,7 +241,6 @@ int inspire_ctx_create(const inspire_params *params, const uint8_t w_seed[32], u
     ctx->gamma = (int)num_columns;
     ctx->dropped = params->approximate_packing_gadget ? 1 : 0;
     ctx->rows = 3 - ctx->dropped;
-    memcpy(ctx->w_seed, w_seed, 32);
 
     const uint64_t two_n = 2ULL * n;
     uint64_t psi = find_psi(two_n);
@@ -268,35 +271,63 @@ int inspire_ctx_create(const inspire_params *params, const uint8_t w_seed[32], u
     hipError_t e = hipGetDevice(&ctx->device);
     if (e == hipSuccess) e = scratch_setup(ctx->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    DevBuf d_seed;
     if (e == hipSuccess) e = ctx->psi.alloc(n * 8);
     if (e == hipSuccess) e = ctx->psi_inv.alloc(n * 8);
     if (e == hipSuccess) e = ctx->gpow.alloc(num_columns * 4);
     if (e == hipSuccess) e = ctx->gpow_back.alloc(num_columns * 4);
     if (e == hipSuccess) e = ctx->twf.alloc(n * sizeof(Pair));
     if (e == hipSuccess) e = ctx->twi.alloc(n * sizeof(Pair));
-    if (e == hipSuccess) e = ctx->wprime.alloc((size_t)ctx->rows * n * sizeof(Pair));
-    if (e == hipSuccess) e = d_seed.alloc(32);
     if (e == hipSuccess) e = hipMemcpy(ctx->psi.p, fwd.data(), n * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(ctx->psi_inv.p, inv.data(), n * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(ctx->twf.p, fwd_p.data(), n * sizeof(Pair), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(ctx->twi.p, inv_p.data(), n * sizeof(Pair), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(ctx->gpow.p, gpow.data(), num_columns * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(ctx->gpow_back.p, gback.data(), num_columns * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_seed.p, w_seed, 32, hipMemcpyHostToDevice);
-    // kept_mask_rows (inspiring2.rs:839-855): stream 0 for the exact gadget, gamma otherwise
-    if (e == hipSuccess)
-        e = launch_expand_mask(ctx->logn, ctx->rows, d_seed.as<uint8_t>(), ctx->dropped ? num_columns : 0,
-                               ctx->dropped, ctx->psi.as<uint64_t>(), ctx->inv_n, ctx->wprime.as<Pair>(), ctx->stream);
-    if (e == hipSuccess && ctx->rows == 3)
-        e = launch_fold_rows(ctx->wprime.as<Pair>(), (int)n, ctx->kx.w, ctx->k0.w, ctx->k1.w, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         set_err(err, err_cap, "HIP failure creating the context: %s", hipGetErrorString(e));
         delete ctx;
         return INSPIRE_ERR_DEVICE;
     }
+    if (w_seed) {
+        int rc = inspire_ctx_attach_crs(ctx, w_seed, err, err_cap);
+        if (rc) {
+            delete ctx;
+            return rc;
+        }
+    }
     *out = ctx;
+    return INSPIRE_OK;
+}
+
+int inspire_ctx_attach_crs(inspire_ctx *ctx, const uint8_t w_seed[32], char *err, size_t err_cap) {
+    if (!ctx || !w_seed) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->has_crs) {
+        if (memcmp(ctx->w_seed, w_seed, 32) == 0) return INSPIRE_OK;
+        set_err(err, err_cap,
+                "context is already bound to another CRS (w seed): its offline packing keys and every seat "
+                "registered against them belong to that CRS; create a context for the new one");
+        return INSPIRE_ERR_INVALID;
+    }
+    const size_t n = ctx->n;
+    DevBuf d_seed;
+    hipError_t e = ctx->wprime.alloc((size_t)ctx->rows * n * sizeof(Pair));
+    if (e == hipSuccess) e = d_seed.alloc(32);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_seed.p, w_seed, 32, hipMemcpyHostToDevice, ctx->stream);
+    // kept_mask_rows (inspiring2.rs:839-855): stream 0 for the exact gadget, gamma otherwise
+    if (e == hipSuccess)
+        e = launch_expand_mask(ctx->logn, ctx->rows, d_seed.as<uint8_t>(), ctx->dropped ? ctx->gamma : 0,
+                               ctx->dropped, ctx->psi.as<uint64_t>(), ctx->inv_n, ctx->wprime.as<Pair>(), ctx->stream);
+    if (e == hipSuccess && ctx->rows == 3)
+        e = launch_fold_rows(ctx->wprime.as<Pair>(), (int)n, ctx->kx.w, ctx->k0.w, ctx->k1.w, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        set_err(err, err_cap, "HIP failure deriving the offline packing keys: %s", hipGetErrorString(e));
+        return INSPIRE_ERR_DEVICE;
+    }
+    memcpy(ctx->w_seed, w_seed, 32);
+    ctx->has_crs = true;
     return INSPIRE_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <atomic>
 #include <mutex>
 #include <new>
 #include <string>
@@ -132,10 +133,16 @@ struct inspire_workspace {
     KernelTrace trace;  // scan and chain events of the next device-buffer launches, when armed
 };
 
+// Refusal of a respond or server over a context created without a CRS.
+constexpr const char *kNoCrs =
+    "context has no CRS: it was created without a w seed (shards and packing stores only); bind the CRS with "
+    "inspire_ctx_attach_crs before responding";
+
 struct inspire_ctx {
     std::mutex mu;
     int logn = 0, n = 0, gamma = 0, rows = 0, dropped = 0;
     uint8_t w_seed[32];
+    std::atomic<bool> has_crs{false};  // wprime derived from w_seed (inspire_ctx_attach_crs)
     DevBuf psi, psi_inv, twf, twi, wprime, gpow, gpow_back, shard_table, col_scratch;
     uint64_t inv_n = 0, inv_n_gamma = 0;
     Pair kx{0, 0}, k0{0, 0}, k1{0, 0};

@@ -299,6 +299,10 @@ int inspire_server_create(inspire_ctx *ctx, size_t max_launch_queries, size_t hb
         set_err(err, err_cap, "server needs a context and a launch size of 1 to 4194304 queries");
         return INSPIRE_ERR_INVALID;
     }
+    if (!ctx->has_crs) {
+        set_err(err, err_cap, kNoCrs);
+        return INSPIRE_ERR_INVALID;
+    }
     inspire_server *sv = new (std::nothrow) inspire_server();
     if (!sv) { set_err(err, err_cap, "out of host memory"); return INSPIRE_ERR_DEVICE; }
     sv->ctx = ctx;

@@ -64,6 +64,7 @@ def lib():
         cp = ctypes.c_char_p
         pp = ctypes.POINTER(ctypes.c_void_p)
         L.inspire_ctx_create.argtypes = [ctypes.POINTER(InspireParams), vp, u32, pp, cp, sz]
+        L.inspire_ctx_attach_crs.argtypes = [vp, vp, cp, sz]
         L.inspire_ctx_destroy.argtypes = [vp]
         L.inspire_ctx_key_rows.argtypes = [vp]
         L.inspire_ctx_key_rows.restype = u32
@@ -222,14 +223,22 @@ class Context:
     """Per-CRS packing state resident on the GPU (ServerInspiringCache analogue)."""
 
     def __init__(self, params, w_seed, num_columns):
+        """w_seed None: a context without a CRS, for shards and their packing stores only
+        (ShardData::precompute_packing); attach_crs binds one before any respond."""
         h = ctypes.c_void_p()
-        seed = (ctypes.c_uint8 * 32).from_buffer_copy(bytes(w_seed))
+        seed = None if w_seed is None else (ctypes.c_uint8 * 32).from_buffer_copy(bytes(w_seed))
         _call(lib().inspire_ctx_create, ctypes.byref(params), seed, num_columns, ctypes.byref(h))
         self._h = h
         self.n = params.ring_dim
         self.gamma = num_columns
         self.key_rows = lib().inspire_ctx_key_rows(h)
         self.shards = {}
+
+    def attach_crs(self, w_seed):
+        """ServerInspiringCache::new(&crs, &encoded) over a context created without a CRS."""
+        seed = (ctypes.c_uint8 * 32).from_buffer_copy(bytes(w_seed))
+        _call(lib().inspire_ctx_attach_crs, self._h, seed)
+        return self
 
     def set_queries_per_cu(self, k):
         _call(lib().inspire_ctx_set_queries_per_cu, self._h, k)

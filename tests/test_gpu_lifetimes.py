@@ -159,3 +159,34 @@ def test_seat_memory_reused_behind_launches(gpu):
             assert np.array_equal(got[0][k], wa) and np.array_equal(got[1][k], wb[:gamma]), k
         seat_b.close()
         ws.close()
+
+
+def test_context_without_crs_builds_stores_then_binds_one(gpu):
+    """ShardData::precompute_packing needs PackParams and the NttContext only (src/pir/setup.rs:
+    335-340): a context created without a CRS takes shards and builds their stores, refuses every
+    respond and a server, and after inspire_ctx_attach_crs answers as a context created with the
+    CRS. Binding the same seed again is a no-op; another seed is refused."""
+    rng = np.random.default_rng(34)
+    gamma = 16
+    w_seed = bytes(range(9, 41))
+    cols = rng.integers(0, 1 << 16, (gamma, N), dtype=np.uint64)
+    y = rng.integers(0, Q, (3, N), dtype=np.uint64)
+    b = rng.integers(0, Q, (2, N), dtype=np.uint64)
+    seeds = [bytes(range(k, k + 32)) for k in range(2)]
+    with gpu.Context(gpu.params_d2048(), None, gamma) as ctx:
+        ctx.upload_columns(0, cols).precompute_packing()
+        with pytest.raises(gpu.PirError, match="context has no CRS"):
+            gpu.Server(ctx, max_launch=8)
+        seat = ctx.seat(y)
+        with pytest.raises(gpu.PirError, match="context has no CRS"):
+            ctx.respond_seeded_batch(seat, [0, 0], seeds, b)
+        ctx.attach_crs(w_seed)
+        ctx.attach_crs(w_seed)
+        with pytest.raises(gpu.PirError, match="already bound to another CRS"):
+            ctx.attach_crs(bytes(32))
+        got = ctx.respond_seeded_batch(seat, [0, 0], seeds, b)
+        p = oracle.Pack(N, gamma)
+        for k in range(2):
+            wa, wb = p.respond(w_seed, oracle.expand_seed(seeds[k], N), b[k], cols, y)
+            assert np.array_equal(got[0][k], wa) and np.array_equal(got[1][k], wb[:gamma]), k
+        seat.close()
