@@ -6,7 +6,9 @@
 // batch under one registered seat) is queued; a worker thread merges whatever is
 // queued, across seats and batch sizes, into one device launch of up to
 // max_launch queries (each query carries its own seat pointer), and keeps up to two
-// launches on the GPU: requests that arrive while one launch runs form the next. A
+// launches on the GPU, each on its own stream so that they overlap (the next launch's scan
+// and first chains run while the previous one's last chains drain): requests that arrive
+// while one launch runs form the next. A
 // request's queries are never split across launches, and every query's result is
 // the same bytes it gets alone (the kernels read nothing across queries).
 //
@@ -58,6 +60,9 @@ struct Slot {
     // inputs copied (copy-in stream), kernels done (compute stream), outputs copied (copy-out
     // stream): the last one completes the slot
     hipEvent_t in_ev = nullptr, kern_ev = nullptr, done_ev = nullptr;
+    // the slot's kernels run here: the two slots' launches overlap on the device, so one
+    // launch's scan and its first chains fill the CUs the other's last chains leave idle
+    hipStream_t compute = nullptr;
     std::vector<Request> reqs;
     size_t count = 0;
     bool busy = false;
@@ -77,10 +82,10 @@ struct inspire_server {
     size_t max_launch = 0;
     size_t hbm_budget = 0;     // 0 = free device memory only
     size_t resident = 0;       // seat bytes admitted
-    // Kernels run on `stream`; a slot's inputs go up on `copy_in` and its outputs come back
-    // on `copy_out`, so the copies of one launch overlap the kernels of its neighbours. All
-    // three are non-blocking streams at the default priority.
-    hipStream_t stream = nullptr, copy_in = nullptr, copy_out = nullptr;
+    // Kernels run on each slot's compute stream; a slot's inputs go up on `copy_in` and its
+    // outputs come back on `copy_out`, so the copies of one launch overlap the kernels of its
+    // neighbours. All are non-blocking streams at the default priority.
+    hipStream_t copy_in = nullptr, copy_out = nullptr;
     Slot slots[2];
 
     std::mutex mu;
@@ -154,7 +159,7 @@ int inspire_server::launch(Slot &s, char *err, size_t err_cap) {
     HIP_TRY(hipMemcpyAsync(s.d_b.p, s.h_b, q * n * 8, hipMemcpyHostToDevice, copy_in));
     HIP_TRY(hipMemcpyAsync(s.d_seats.p, s.h_seats, q * sizeof(void *), hipMemcpyHostToDevice, copy_in));
     HIP_TRY(hipEventRecord(s.in_ev, copy_in));
-    HIP_TRY(hipStreamWaitEvent(stream, s.in_ev, 0));
+    HIP_TRY(hipStreamWaitEvent(s.compute, s.in_ev, 0));
     // seats registered since their kernels were last seen complete: this launch waits for them
     for (const Request &r : s.reqs) {
         inspire_seat *st = r.seat.get();
@@ -162,7 +167,7 @@ int inspire_server::launch(Slot &s, char *err, size_t err_cap) {
         if (hipEventQuery(st->ready) == hipSuccess)
             st->settled = true;
         else
-            HIP_TRY(hipStreamWaitEvent(stream, st->ready, 0));
+            HIP_TRY(hipStreamWaitEvent(s.compute, st->ready, 0));
     }
     BatchDev bd;
     bd.count = (int)q;
@@ -185,9 +190,9 @@ int inspire_server::launch(Slot &s, char *err, size_t err_cap) {
         bd.first = g + gc;
         bd.ngroups = g + 2 * gc + 1;
     }
-    HIP_TRY(launch_respond(dev, bd, stream));
-    HIP_TRY(ctx->fence.note(stream));
-    HIP_TRY(hipEventRecord(s.kern_ev, stream));
+    HIP_TRY(launch_respond(dev, bd, s.compute));
+    HIP_TRY(ctx->fence.note(s.compute));
+    HIP_TRY(hipEventRecord(s.kern_ev, s.compute));
     HIP_TRY(hipStreamWaitEvent(copy_out, s.kern_ev, 0));
     HIP_TRY(hipMemcpyAsync(s.h_a, s.d_a.p, q * n * 8, hipMemcpyDeviceToHost, copy_out));
     HIP_TRY(hipMemcpyAsync(s.h_bp, s.d_bp.p, q * g * 8, hipMemcpyDeviceToHost, copy_out));
@@ -271,7 +276,7 @@ void inspire_server::run() {
         if (rc != INSPIRE_OK) {
             // a partly enqueued launch may still read or write the slot's buffers
             (void)hipStreamSynchronize(copy_in);
-            (void)hipStreamSynchronize(stream);
+            (void)hipStreamSynchronize(s.compute);
             (void)hipStreamSynchronize(copy_out);
             std::lock_guard<std::mutex> lk(mu);
             for (Request &r : s.reqs) {
@@ -309,10 +314,10 @@ int inspire_server_create(inspire_ctx *ctx, size_t max_launch_queries, size_t hb
     sv->max_launch = max_launch_queries;
     sv->hbm_budget = hbm_budget_bytes;
     auto drop_streams = [sv] {
-        for (hipStream_t st : {sv->stream, sv->copy_in, sv->copy_out})
+        for (hipStream_t st : {sv->slots[0].compute, sv->slots[1].compute, sv->copy_in, sv->copy_out})
             if (st) (void)hipStreamDestroy(st);
     };
-    for (hipStream_t *st : {&sv->stream, &sv->copy_in, &sv->copy_out}) {
+    for (hipStream_t *st : {&sv->slots[0].compute, &sv->slots[1].compute, &sv->copy_in, &sv->copy_out}) {
         hipError_t e = hipStreamCreateWithFlags(st, hipStreamNonBlocking);
         if (e != hipSuccess) {
             set_err(err, err_cap, "HIP failure creating the server streams: %s", hipGetErrorString(e));
@@ -342,7 +347,7 @@ void inspire_server_destroy(inspire_server *sv) {
     }
     sv->cv_work.notify_all();
     sv->worker.join();
-    for (hipStream_t st : {sv->copy_in, sv->stream, sv->copy_out}) {
+    for (hipStream_t st : {sv->copy_in, sv->slots[0].compute, sv->slots[1].compute, sv->copy_out}) {
         (void)hipStreamSynchronize(st);
         (void)hipStreamDestroy(st);
     }

@@ -58,6 +58,8 @@ struct Slot {
     // inputs copied (copy-in stream), kernel done (compute stream), outputs copied (copy-out
     // stream): the last one completes the slot
     hipEvent_t in_ev = nullptr, kern_ev = nullptr, done_ev = nullptr;
+    // the slot's kernel runs here: the two slots' launches overlap on the device
+    hipStream_t compute = nullptr;
     std::vector<Request> reqs;
     bool busy = false;
     ~Slot() {
@@ -75,10 +77,10 @@ struct inspire_sm_batcher {
     inspire_sm_server *srv = nullptr;
     size_t max_launch = 0, hbm_budget = 0, resident = 0;
     size_t n = 0, nb = 0;
-    // The kernel runs on `stream`; a slot's inputs go up on `copy_in` and its outputs come
-    // back on `copy_out`, so one launch's copies overlap its neighbours' kernels. All three
-    // are non-blocking streams at the default priority.
-    hipStream_t stream = nullptr, copy_in = nullptr, copy_out = nullptr;
+    // The kernel runs on each slot's compute stream; a slot's inputs go up on `copy_in` and
+    // its outputs come back on `copy_out`, so one launch's copies overlap its neighbours'
+    // kernels. All are non-blocking streams at the default priority.
+    hipStream_t copy_in = nullptr, copy_out = nullptr;
     Slot slots[2];
 
     std::mutex mu;
@@ -135,7 +137,7 @@ int inspire_sm_batcher::launch(Slot &s, char *err, size_t err_cap) {
     SM_TRY(hipMemcpyAsync(s.d_seeds.p, s.h_seeds, q * 32, hipMemcpyHostToDevice, copy_in));
     SM_TRY(hipMemcpyAsync(s.d_bodies.p, s.h_bodies, q * n * 4, hipMemcpyHostToDevice, copy_in));
     SM_TRY(hipEventRecord(s.in_ev, copy_in));
-    SM_TRY(hipStreamWaitEvent(stream, s.in_ev, 0));
+    SM_TRY(hipStreamWaitEvent(s.compute, s.in_ev, 0));
     // seats registered since their kernel was last seen complete: this launch waits for them
     for (const Request &r : s.reqs) {
         inspire_sm_seat *st = r.seat.get();
@@ -143,17 +145,17 @@ int inspire_sm_batcher::launch(Slot &s, char *err, size_t err_cap) {
         if (hipEventQuery(st->ready) == hipSuccess)
             st->settled = true;
         else
-            SM_TRY(hipStreamWaitEvent(stream, st->ready, 0));
+            SM_TRY(hipStreamWaitEvent(s.compute, st->ready, 0));
     }
     SmBatchDev bd{(int)q, s.d_jobs.as<SmJob>(), s.d_seeds.as<uint8_t>(), s.d_bodies.as<uint32_t>(),
                   s.d_mask.as<uint32_t>(), s.d_body.as<uint32_t>()};
     {
         // a fence event behind the kernel, so destroying a seat or table of the server waits for it
         std::lock_guard<std::mutex> lk(srv->mu);
-        SM_TRY(sm_launch_respond(srv->dev, bd, stream));
-        SM_TRY(srv->fence.note(stream));
+        SM_TRY(sm_launch_respond(srv->dev, bd, s.compute));
+        SM_TRY(srv->fence.note(s.compute));
     }
-    SM_TRY(hipEventRecord(s.kern_ev, stream));
+    SM_TRY(hipEventRecord(s.kern_ev, s.compute));
     SM_TRY(hipStreamWaitEvent(copy_out, s.kern_ev, 0));
     SM_TRY(hipMemcpyAsync(s.h_mask, s.d_mask.p, q * n * 4, hipMemcpyDeviceToHost, copy_out));
     SM_TRY(hipMemcpyAsync(s.h_body, s.d_body.p, q * nb * 4, hipMemcpyDeviceToHost, copy_out));
@@ -218,7 +220,7 @@ void inspire_sm_batcher::run() {
         if (rc != INSPIRE_OK) {
             // a partly enqueued launch may still read or write the slot's buffers
             (void)hipStreamSynchronize(copy_in);
-            (void)hipStreamSynchronize(stream);
+            (void)hipStreamSynchronize(s.compute);
             (void)hipStreamSynchronize(copy_out);
             std::lock_guard<std::mutex> lk(mu);
             for (Request &r : s.reqs) {
@@ -254,10 +256,10 @@ int inspire_sm_batcher_create(inspire_sm_server *srv, size_t max_launch_queries,
     b->n = srv->params.ring_dim;
     b->nb = 2 * (size_t)srv->params.record_bytes;
     auto drop_streams = [b] {
-        for (hipStream_t st : {b->stream, b->copy_in, b->copy_out})
+        for (hipStream_t st : {b->slots[0].compute, b->slots[1].compute, b->copy_in, b->copy_out})
             if (st) (void)hipStreamDestroy(st);
     };
-    for (hipStream_t *st : {&b->stream, &b->copy_in, &b->copy_out}) {
+    for (hipStream_t *st : {&b->slots[0].compute, &b->slots[1].compute, &b->copy_in, &b->copy_out}) {
         hipError_t e = hipStreamCreateWithFlags(st, hipStreamNonBlocking);
         if (e != hipSuccess) {
             sm_err(err, err_cap, "HIP failure creating the batcher streams: %s", hipGetErrorString(e));
@@ -287,7 +289,7 @@ void inspire_sm_batcher_destroy(inspire_sm_batcher *b) {
     }
     b->cv_work.notify_all();
     b->worker.join();
-    for (hipStream_t st : {b->copy_in, b->stream, b->copy_out}) {
+    for (hipStream_t st : {b->copy_in, b->slots[0].compute, b->slots[1].compute, b->copy_out}) {
         (void)hipStreamSynchronize(st);
         (void)hipStreamDestroy(st);
     }
