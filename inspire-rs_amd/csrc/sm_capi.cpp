@@ -417,17 +417,42 @@ int inspire_sm_respond(inspire_sm_server *s, const inspire_sm_table *table, cons
 // batch launches with nothing but the kernel on the stream. A launch of this server on any
 // stream may still read the old list, so a change waits for the server's launches enqueued so
 // far (its fence), not for the whole device. Caller holds s->mu.
-static int upload_device_jobs(inspire_sm_server *s, const std::vector<SmJob> &jobs, char *err, size_t err_cap) {
-    if (jobs.size() == s->djobs_host.size() &&
-        std::memcmp(jobs.data(), s->djobs_host.data(), jobs.size() * sizeof(SmJob)) == 0)
-        return INSPIRE_OK;
-    SM_TRY(wait_for(s->fence.snapshot()));
-    if (jobs.size() > s->djobs_cap) {
-        SM_TRY(s->djobs.alloc(jobs.size() * sizeof(SmJob)));
-        s->djobs_cap = jobs.size();
+// The device job list of a launch on stream st (caller holds s->mu): the newest list when it
+// was written on st and the handles repeat it, else the next list of the ring, rewritten on st
+// behind the last launch that read it (all on one stream, so the last one is the latest).
+// *used receives the list's event, which the caller records behind its launch.
+static int upload_device_jobs(inspire_sm_server *s, const std::vector<SmJob> &jobs, hipStream_t st,
+                              const SmJob **dev_jobs, hipEvent_t *used, char *err, size_t err_cap) {
+    if (s->djobs_last >= 0) {
+        inspire_sm_server::JobList &l = s->djobs[s->djobs_last];
+        if (l.stream == st && jobs.size() == l.host.size() &&
+            std::memcmp(jobs.data(), l.host.data(), jobs.size() * sizeof(SmJob)) == 0) {
+            // ordered behind the list's copy already (same stream); the wait only matters for a
+            // stream created with a destroyed one's handle
+            SM_TRY(hipStreamWaitEvent(st, l.used, 0));
+            *dev_jobs = l.buf.as<SmJob>();
+            *used = l.used;
+            return INSPIRE_OK;
+        }
     }
-    SM_TRY(hipMemcpy(s->djobs.p, jobs.data(), jobs.size() * sizeof(SmJob), hipMemcpyHostToDevice));
-    s->djobs_host = jobs;
+    const int k = s->djobs_next;
+    inspire_sm_server::JobList &l = s->djobs[k];
+    if (!l.used) SM_TRY(hipEventCreateWithFlags(&l.used, hipEventDisableTiming));
+    if (jobs.size() > l.cap) {
+        // a larger list: the old buffer may still be read by launches in flight
+        SM_TRY(wait_for(s->fence.snapshot()));
+        SM_TRY(l.buf.alloc(jobs.size() * sizeof(SmJob)));
+        l.cap = jobs.size();
+    } else {
+        SM_TRY(hipStreamWaitEvent(st, l.used, 0));
+    }
+    SM_TRY(hipMemcpyAsync(l.buf.p, jobs.data(), jobs.size() * sizeof(SmJob), hipMemcpyHostToDevice, st));
+    l.host = jobs;
+    l.stream = st;
+    s->djobs_last = k;
+    s->djobs_next = (k + 1) % inspire_sm_server::kJobLists;
+    *dev_jobs = l.buf.as<SmJob>();
+    *used = l.used;
     return INSPIRE_OK;
 }
 
@@ -511,12 +536,15 @@ int inspire_sm_respond_batch_device_wire(inspire_sm_server *s, size_t count, con
     if (rc) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
     std::lock_guard<std::mutex> lk(s->mu);
-    rc = upload_device_jobs(s, jobs, err, err_cap);
+    const SmJob *dj = nullptr;
+    hipEvent_t used = nullptr;
+    rc = upload_device_jobs(s, jobs, st, &dj, &used, err, err_cap);
     if (rc) return rc;
-    SmBatchDev b{(int)count, s->djobs.as<SmJob>(), d_seeds, d_bodies, d_scratch, nullptr};
+    SmBatchDev b{(int)count, dj, d_seeds, d_bodies, d_scratch, nullptr};
     b.wire_out = d_wire_out;
     b.wire_stride = wire_stride;
     SM_TRY(launch_traced(s, b, st));
+    SM_TRY(hipEventRecord(used, st));
     SM_TRY(s->fence.note(st));
     return INSPIRE_OK;
 }
@@ -539,11 +567,14 @@ int inspire_sm_respond_batch_device(inspire_sm_server *s, size_t count, const in
     if (rc) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
     std::lock_guard<std::mutex> lk(s->mu);
-    rc = upload_device_jobs(s, jobs, err, err_cap);
+    const SmJob *dj = nullptr;
+    hipEvent_t used = nullptr;
+    rc = upload_device_jobs(s, jobs, st, &dj, &used, err, err_cap);
     if (rc) return rc;
-    SmBatchDev b{(int)count, s->djobs.as<SmJob>(), d_seeds, d_bodies, d_mask_out, d_body_out};
+    SmBatchDev b{(int)count, dj, d_seeds, d_bodies, d_mask_out, d_body_out};
     if (!kernel_ms) {
         SM_TRY(launch_traced(s, b, st));
+        SM_TRY(hipEventRecord(used, st));
         SM_TRY(s->fence.note(st));
         return INSPIRE_OK;
     }
@@ -553,6 +584,7 @@ int inspire_sm_respond_batch_device(inspire_sm_server *s, size_t count, const in
     hipError_t e = hipEventRecord(ev[0], st);
     if (e == hipSuccess) e = sm_launch_respond(s->dev, b, st);
     if (e == hipSuccess) e = hipEventRecord(ev[1], st);
+    if (e == hipSuccess) e = hipEventRecord(used, st);
     if (e == hipSuccess) e = s->fence.note(st);
     if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
     if (e == hipSuccess) e = hipEventElapsedTime(kernel_ms, ev[0], ev[1]);

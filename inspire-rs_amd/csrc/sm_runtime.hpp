@@ -71,10 +71,20 @@ struct inspire_sm_server {
     // staging of the host-buffer entry points
     inspire_sm::Buf jobs, seeds, bodies, mask, body;
     size_t cap = 0;
-    // job list of the device entry point, reused while the same handles repeat
-    inspire_sm::Buf djobs;
-    std::vector<inspire::SmJob> djobs_host;
-    size_t djobs_cap = 0;
+    // Job lists of the device entry points: a ring, so that launches on several streams can be
+    // in flight at once. A list is written and read on one stream, reused by the next launch on
+    // that stream while the same handles repeat, and rewritten only behind the last launch that
+    // read it (its event, waited for on the device by the stream of the launch that rewrites it).
+    struct JobList {
+        inspire_sm::Buf buf;
+        size_t cap = 0;
+        hipEvent_t used = nullptr;  // recorded behind the last launch that read buf
+        hipStream_t stream = nullptr;  // the stream buf was written and read on
+        std::vector<inspire::SmJob> host;
+    };
+    static constexpr int kJobLists = 4;
+    JobList djobs[kJobLists];
+    int djobs_next = 0, djobs_last = -1;
     // staged response bytes of the host wire entry point
     inspire_sm::Buf wire;
     size_t wire_cap = 0;
@@ -95,6 +105,8 @@ struct inspire_sm_server {
             (void)hipStreamDestroy(stream);
         }
         (void)inspire::wait_for(fence.snapshot());
+        for (JobList &j : djobs)
+            if (j.used) (void)hipEventDestroy(j.used);
     }
     std::vector<hipEvent_t> launches_so_far() {
         std::lock_guard<std::mutex> lk(mu);
