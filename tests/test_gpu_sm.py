@@ -272,3 +272,55 @@ def test_device_wire_bytes_equal_the_host_codec(gpu, d, gamma):
                                              d_wire.data_ptr(), size + 1)
         for x in tables + seats:
             x.close()
+
+
+def test_device_launches_on_two_streams_with_their_own_job_lists(gpu):
+    """Device launches alternating between two streams, each naming its own tables and keys (so
+    each uploads its own job list) with no wait between them: more launches than the server's
+    ring of job lists, a launch that repeats its predecessor's handles on the other stream and one
+    that repeats them on the same stream. Every
+    launch equals the host-buffer path's answer to the same queries."""
+    import torch
+
+    rng = np.random.default_rng(8)
+    dev = torch.device("cuda:0")
+    L, n, launches = 24, 2048, 8
+    with gpu.SmServer(gpu.ring2048_cell256()) as server:
+        tables = [server.table(oracle.sm_kat_cells(500 + i, 2048).tobytes()) for i in range(3)]
+        keys = [gpu.PackingKey(2, bytes(rng.integers(0, 256, 32, dtype=np.uint8)),
+                               rng.integers(0, SM_Q, n, dtype=np.uint32), rng.integers(0, SM_P, n, dtype=np.uint32))
+                for _ in range(3)]
+        seats = [server.seat(k) for k in keys]
+        # launch i runs on stream i % 2, the last one on the stream of the one before it; launch 2
+        # repeats launch 1's handles on the other stream, the last repeats its predecessor's on
+        # the same stream (the reused job list)
+        on = [i % 2 for i in range(launches - 1)] + [(launches - 2) % 2]
+        picks = []
+        for i in range(launches):
+            if i in (2, launches - 1):
+                picks.append(picks[i - 1])
+                continue
+            picks.append(([tables[j] for j in rng.integers(0, 3, L)], [seats[j] for j in rng.integers(0, 3, L)]))
+        queries = [[gpu.Query(bytes(rng.integers(0, 256, 32, dtype=np.uint8)),
+                              rng.integers(0, SM_Q, n, dtype=np.uint32)) for _ in range(L)] for _ in range(launches)]
+        d_seeds = [torch.from_numpy(np.frombuffer(b"".join(q.seed for q in qs), np.uint8).copy()).to(dev)
+                   for qs in queries]
+        d_bodies = [torch.from_numpy(np.stack([q.body for q in qs]).astype(np.uint32).view(np.int32)).to(dev)
+                    for qs in queries]
+        masks = [torch.empty((L, n), dtype=torch.int32, device=dev) for _ in range(launches)]
+        bodies = [torch.empty((L, 2 * 246), dtype=torch.int32, device=dev) for _ in range(launches)]
+        streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+        torch.cuda.synchronize()
+        print(f"sm: {launches} device launches on two streams", flush=True)
+        for i in range(launches):
+            server.respond_batch_device(picks[i][0], picks[i][1], d_seeds[i].data_ptr(), d_bodies[i].data_ptr(),
+                                        masks[i].data_ptr(), bodies[i].data_ptr(), stream=streams[on[i]].cuda_stream)
+        torch.cuda.synchronize()
+        for i in range(launches):
+            want_m, want_b = server.respond_batch(picks[i][0], picks[i][1], queries[i])
+            assert np.array_equal(masks[i].cpu().numpy().view(np.uint32), want_m), i
+            assert np.array_equal(bodies[i].cpu().numpy().view(np.uint32), want_b), i
+        for s in seats:
+            s.close()
+        for t in tables:
+            t.close()
