@@ -42,7 +42,7 @@ typedef struct inspire_workspace inspire_workspace;
 
 /* The fields of InspireParams the respond path reads (src/params.rs:120-190). */
 typedef struct {
-    uint32_t ring_dim;                    /* 256 or 2048 */
+    uint32_t ring_dim;                    /* 256, 2048 or 4096 (InspireParams::d4096) */
     uint64_t q;                           /* must be INSPIRE_Q */
     uint64_t gadget_base;                 /* must be 2^20 */
     uint32_t packing_gadget_len;          /* must be 3 */

@@ -584,6 +584,7 @@ hipError_t launch_chain_only(const CtxDev &c, const BatchDev &b, hipStream_t s) 
     switch (c.logn) {
         case 8: return launch_chain_t<8>(c, b, s);
         case 11: return launch_chain_t<11>(c, b, s);
+        case 12: return launch_chain_t<12>(c, b, s);
         default: return hipErrorInvalidValue;
     }
 }

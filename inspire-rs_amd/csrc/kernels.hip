@@ -500,6 +500,7 @@ hipError_t launch_expand_mask(int logn, int rows, const uint8_t *seed, uint64_t 
     switch (logn) {
         case 8: hipLaunchKernelGGL(k_expand_mask<8>, dim3(rows), dim3(kThreads), 0, s, seed, stream, dropped, psi, inv_n, out); break;
         case 11: hipLaunchKernelGGL(k_expand_mask<11>, dim3(rows), dim3(kThreads), 0, s, seed, stream, dropped, psi, inv_n, out); break;
+        case 12: hipLaunchKernelGGL(k_expand_mask<12>, dim3(rows), dim3(kThreads), 0, s, seed, stream, dropped, psi, inv_n, out); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -510,6 +511,7 @@ hipError_t launch_seat(int logn, int rows, const uint64_t *y, const uint64_t *ps
     switch (logn) {
         case 8: hipLaunchKernelGGL(k_seat<8>, dim3(rows), dim3(kThreads), 0, s, y, psi, inv_n, out); break;
         case 11: hipLaunchKernelGGL(k_seat<11>, dim3(rows), dim3(kThreads), 0, s, y, psi, inv_n, out); break;
+        case 12: hipLaunchKernelGGL(k_seat<12>, dim3(rows), dim3(kThreads), 0, s, y, psi, inv_n, out); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -523,6 +525,7 @@ hipError_t launch_store_build(const CtxDev &c, const uint16_t *records, uint16_t
     switch (c.logn) {
         case 8: hipLaunchKernelGGL(k_store_build<8>, dim3(c.gamma), dim3(kThreads), 0, s, c, col_scratch, store); break;
         case 11: hipLaunchKernelGGL(k_store_build<11>, dim3(c.gamma), dim3(kThreads), 0, s, c, col_scratch, store); break;
+        case 12: hipLaunchKernelGGL(k_store_build<12>, dim3(c.gamma), dim3(kThreads), 0, s, c, col_scratch, store); break;
         default: return hipErrorInvalidValue;
     }
     hipLaunchKernelGGL(k_column_tags, dim3(c.gamma), dim3(kThreads), 0, s, col_scratch, tag_key, c.n, tags);

@@ -36,7 +36,7 @@ def test_chacha20_rfc7539_block():
     assert list(out) == want
 
 
-@pytest.mark.parametrize("n", [16, 256, 2048])
+@pytest.mark.parametrize("n", [16, 256, 2048, 4096])
 def test_ntt_is_negacyclic_convolution(n):
     rng = np.random.default_rng(n)
     a = rng.integers(0, Q, n, dtype=np.uint64)
@@ -75,7 +75,7 @@ def test_montgomery_equals_plain_product():
         assert ntt.mont_mul(a, b) == a * b * rinv % Q
 
 
-@pytest.mark.parametrize("n,gamma", [(256, 16), (2048, 256)])
+@pytest.mark.parametrize("n,gamma", [(256, 16), (2048, 256), (4096, 256)])
 def test_automorph_tables_invariant_and_closed_form(n, gamma):
     p = oracle.Pack(n, gamma)
     ntt = oracle.Ntt(n)
@@ -133,6 +133,22 @@ def test_store_path_equals_per_query_path():
     x = p.respond(fx["w_seed"], fx["a"], fx["b"], fx["columns"], fx["y_body"])
     y = p.respond(fx["w_seed"], fx["a"], fx["b"], fx["columns"], fx["y_body"], S)
     assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
+
+
+def test_store_path_equals_per_query_path_at_ring_4096():
+    # InspireParams::d4096 (params.rs:392-436): the same invariant on the wider ring, for both
+    # gadgets
+    rng = np.random.default_rng(4096)
+    n, gamma = 4096, 16
+    cols = rng.integers(0, 1 << 16, (gamma, n), dtype=np.uint64)
+    b = rng.integers(0, Q, n, dtype=np.uint64)
+    a = oracle.expand_seed(bytes(range(32)), n)
+    for dropped in (0, 1):
+        p = oracle.Pack(n, gamma, 3, dropped)
+        y = rng.integers(0, Q, (3 - dropped, n), dtype=np.uint64)
+        x = p.respond(bytes(32), a, b, cols, y)
+        z = p.respond(bytes(32), a, b, cols, y, p.store_build(cols))
+        assert np.array_equal(x[0], z[0]) and np.array_equal(x[1], z[1])
 
 
 def test_fold_b_is_coefficient_zero_of_products():
