@@ -271,7 +271,7 @@ typedef struct inspire_sm_seat inspire_sm_seat;
 
 /* special_modulus::Params (src/special_modulus/mod.rs:62-97). */
 typedef struct {
-    uint32_t ring_dim;          /* 2048 served; the gfx950 kernels run 64..2048 */
+    uint32_t ring_dim;          /* 2048 served; the gfx950 kernels run 16..8192 */
     uint32_t gamma;             /* 512 served */
     uint64_t q;                 /* INSPIRE_SM_Q served; at most 2^32 / 6 here (lazy residues) */
     uint64_t special_prime;     /* INSPIRE_SM_P served */

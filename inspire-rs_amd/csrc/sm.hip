@@ -332,7 +332,13 @@ __device__ __forceinline__ int rot_slot(uint32_t t, int j, int logn) {
 // the seat rows the chain re-reads each step come from L2, and four unspilled workgroups per CU
 // beat five with 17 registers spilled (23.6-24.5 against 25.9-26.1 ms per launch of 2560).
 template <int LOGN>
-__global__ __launch_bounds__((1 << LOGN) / 8, 4) void k_sm_respond(SmCtxDev c, SmBatchDev bt) {
+struct SmOccupancy {
+    // workgroups per CU the register budget is set for: 4 waves per SIMD at every ring up to
+    // 2048 (n/8 <= 256 threads), and at 4096 and 8192 (512 and 1024 threads per workgroup)
+    static constexpr int blocks = LOGN <= 11 ? 4 : LOGN == 12 ? 2 : 1;
+};
+template <int LOGN>
+__global__ __launch_bounds__((1 << LOGN) / 8, SmOccupancy<LOGN>::blocks) void k_sm_respond(SmCtxDev c, SmBatchDev bt) {
     constexpr int N = 1 << LOGN, T = N / 8, NW = Lds<LOGN>::words;
     // three buffers (27 KiB): the transform buffer X also serves the rotation of V (PV), the
     // accumulators keep PQ and PP
@@ -711,12 +717,16 @@ hipError_t respond_t(const SmCtxDev &c, const SmBatchDev &b, hipStream_t s) {
 
 #define SM_DISPATCH(fn, ...)                       \
     switch (c.logn) {                              \
+        case 4: return fn<4>(__VA_ARGS__);         \
+        case 5: return fn<5>(__VA_ARGS__);         \
         case 6: return fn<6>(__VA_ARGS__);         \
         case 7: return fn<7>(__VA_ARGS__);         \
         case 8: return fn<8>(__VA_ARGS__);         \
         case 9: return fn<9>(__VA_ARGS__);         \
         case 10: return fn<10>(__VA_ARGS__);       \
         case 11: return fn<11>(__VA_ARGS__);       \
+        case 12: return fn<12>(__VA_ARGS__);       \
+        case 13: return fn<13>(__VA_ARGS__);       \
         default: return hipErrorInvalidValue;      \
     }
 

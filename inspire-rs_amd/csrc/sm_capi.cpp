@@ -162,8 +162,8 @@ int inspire_sm_server_create(const inspire_sm_params *params, inspire_sm_server 
     const uint64_t n = params->ring_dim, gamma = params->gamma, q = params->q, p = params->special_prime;
     // what the gfx950 kernels run: n/8 threads per query with 8 slots each, lazy
     // residues mod q in [0, 4q), and 4-byte words of nibbles
-    if (n < 64 || n > 2048) {
-        sm_err(err, err_cap, "ring dimension %llu is outside the [64, 2048] the gfx950 kernels run",
+    if (n > 8192) {
+        sm_err(err, err_cap, "ring dimension %llu is outside the [16, 8192] the gfx950 kernels run",
                (unsigned long long)n);
         return INSPIRE_ERR_INVALID;
     }

@@ -86,6 +86,30 @@ def test_transform_multiplies_negacyclically():
         assert np.array_equal(fwd(fa, inverse=True), a)
 
 
+@pytest.mark.parametrize("d,q,p", [(16, SM_Q, SM_P), (4096, SM_Q, 2_147_377_153), (8192, 536_690_689, 2_147_352_577)])
+def test_transform_multiplies_negacyclically_at_the_other_rings(d, q, p):
+    # rings 16 .. 8192 the kernels also run, with primes 1 mod 2d; an exact product with a
+    # sparse operand (three terms, one wrapping past X^d) keeps the big rings fast
+    ring = oracle.SmRing(d=d, gamma=8, q=q, p=p, tables=False)
+    rng = np.random.default_rng(d)
+    for m, fwd in ((q, ring.ntt_q), (p, ring.ntt_p)):
+        a = rng.integers(0, m, d, dtype=np.uint32)
+        b = np.zeros(d, dtype=np.uint32)
+        b[0], b[1], b[d - 1] = 3, m - 1, 7
+        fa, fb = fwd(a), fwd(b)
+        prod = ((fa.astype(object) * fb.astype(object)) % m).astype(np.uint32)
+        want = [0] * d
+        for i in range(d):
+            for j, c in ((0, 3), (1, m - 1), (d - 1, 7)):
+                k, v = i + j, int(a[i]) * c
+                if k >= d:
+                    want[k - d] = (want[k - d] - v) % m
+                else:
+                    want[k] = (want[k] + v) % m
+        assert np.array_equal(fwd(prod, inverse=True), np.array(want, dtype=np.uint32))
+        assert np.array_equal(fwd(fa, inverse=True), a)
+
+
 def _brv(x, bits):
     r = np.zeros_like(x)
     for i in range(bits):
@@ -204,7 +228,7 @@ def _frame_model(ring, table, seat, qseed, qbody):
     return mask, np.array(body, dtype=np.uint32)
 
 
-@pytest.mark.parametrize("d,gamma", [(64, 8), (256, 32)])
+@pytest.mark.parametrize("d,gamma", [(16, 8), (32, 16), (64, 8), (256, 32)])
 def test_rotated_frame_recursion_equals_the_reference_recursion(d, gamma):
     ring = oracle.SmRing(d=d, gamma=gamma)
     rng = np.random.default_rng(d + gamma)
