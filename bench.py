@@ -414,6 +414,42 @@ def sm_cpu_baseline(budget_s):
     return out
 
 
+def sm_two_streams(torch, srv, picks, d_seeds, d_bodies, d_mask, d_body, L, K, wall, qps, lat):
+    """The special-modulus launches alternating between two streams with their own outputs (and
+    job lists): a launch's first workgroups run while the previous launch's last ones drain.
+    Returns the faster pipeline's (queries/s, launch latencies, name) and the two-stream figures."""
+    last_mask = d_mask.clone()
+    st2 = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs2 = [(d_mask, d_body), (torch.empty_like(d_mask), torch.empty_like(d_body))]
+
+    def issue2(i):
+        k = i % 2
+        return srv.respond_batch_device(picks[i][0], picks[i][1], d_seeds[i].data_ptr(), d_bodies[i].data_ptr(),
+                                        outs2[k][0].data_ptr(), outs2[k][1].data_ptr(), stream=st2[k].cuda_stream)
+
+    issue2(0)
+    torch.cuda.synchronize()
+    s2 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+    e2 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+    t0 = time.perf_counter()
+    for k in range(K):
+        s2[k].record(st2[(1 + k) % 2])  # the stream issue2(1 + k) launches on
+        issue2(1 + k)
+        e2[k].record(st2[(1 + k) % 2])
+    torch.cuda.synchronize()
+    wall2 = time.perf_counter() - t0
+    assert torch.equal(outs2[K % 2][0], last_mask), "two-stream launch differs from the stream launch"
+    lat2 = np.array([a_.elapsed_time(b_) for a_, b_ in zip(s2, e2)])
+    two_streams = {"value": round(L * K / wall2, 1), "unit": "queries/s",
+                   "launch_ms": {"p50": round(float(np.percentile(lat2, 50)), 3),
+                                 "p99": round(float(np.percentile(lat2, 99)), 3),
+                                 "unit": "one launch on its stream, overlapping the other stream's"}}
+    pipeline = "stream"
+    if wall2 < wall:
+        pipeline, qps, lat = "two streams", L * K / wall2, lat2
+    return qps, lat, pipeline, two_streams
+
+
 def special_modulus(g, torch, dev, stream, args):
     """Served special-modulus set (ring2048_cell256): 2^22 cells of 256 B in 2048 step-1 tables
     (one per shard of 2048 rows), 256 seated keys with their own mask seeds, batches of queries
@@ -462,37 +498,10 @@ def special_modulus(g, torch, dev, stream, args):
     one_stream = {"value": round(qps, 1), "unit": "queries/s",
                   "launch_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                                 "p99": round(float(np.percentile(lat, 99)), 3)}}
-    # the same launches alternating between two streams with their own outputs (and job lists):
-    # a launch's first workgroups run while the previous launch's last ones drain
-    last_mask = d_mask.clone()
-    st2 = [torch.cuda.Stream(), torch.cuda.Stream()]
-    outs2 = [(d_mask, d_body), (torch.empty_like(d_mask), torch.empty_like(d_body))]
-
-    def issue2(i):
-        k = i % 2
-        return srv.respond_batch_device(picks[i][0], picks[i][1], d_seeds[i].data_ptr(), d_bodies[i].data_ptr(),
-                                        outs2[k][0].data_ptr(), outs2[k][1].data_ptr(), stream=st2[k].cuda_stream)
-
-    issue2(0)
-    torch.cuda.synchronize()
-    s2 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
-    e2 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
-    t0 = time.perf_counter()
-    for k in range(K):
-        s2[k].record(st2[(1 + k) % 2])  # the stream issue2(1 + k) launches on
-        issue2(1 + k)
-        e2[k].record(st2[(1 + k) % 2])
-    torch.cuda.synchronize()
-    wall2 = time.perf_counter() - t0
-    assert torch.equal(outs2[K % 2][0], last_mask), "two-stream launch differs from the stream launch"
-    lat2 = np.array([a_.elapsed_time(b_) for a_, b_ in zip(s2, e2)])
-    two_streams = {"value": round(L * K / wall2, 1), "unit": "queries/s",
-                   "launch_ms": {"p50": round(float(np.percentile(lat2, 50)), 3),
-                                 "p99": round(float(np.percentile(lat2, 99)), 3),
-                                 "unit": "one launch on its stream, overlapping the other stream's"}}
-    pipeline = "stream"
-    if wall2 < wall:
-        pipeline, qps, lat = "two streams", L * K / wall2, lat2
+    two_streams, pipeline = None, "stream"
+    if not args.no_two_streams:
+        qps, lat, pipeline, two_streams = sm_two_streams(torch, srv, picks, d_seeds, d_bodies, d_mask, d_body, L, K,
+                                                         wall, qps, lat)
     mm_s = L * SM_MM_PER_QUERY / (kms * 1e-3)
     # per query the kernel streams its table (gamma x n words) and reads its cells, seat and body
     bytes_per_query = SM_GAMMA * SM_N * 4 + SM_N * SM_CELL + 4 * SM_N * 4 + SM_N * 4 + SM_N * 4 + 2 * SM_RECORD * 4
@@ -820,6 +829,8 @@ def main():
     ap.add_argument("--plain-queries", type=int, default=16)
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="time stream launches only")
+    ap.add_argument("--no-two-streams", action="store_true",
+                    help="time launches on one stream only (no launches alternating between two streams)")
     ap.add_argument("--no-autotune", action="store_true", help="keep --queries-per-cu instead of autotuning")
     ap.add_argument("--no-extra", action="store_true", help="headline only: no 2^18, 2^16 or plain legs")
     ap.add_argument("--served-seconds", type=float, default=4.0)
@@ -887,7 +898,8 @@ def main():
     seat = ctx.seat(y)
     log("bench: headline launches")
     head = batched_leg(g, torch, dev, stream, ctx, seat, args.shards, B, M, args.steps, args.warmup, rng,
-                       tune=not args.no_autotune, graph=not args.no_graph, single=not args.no_extra)
+                       tune=not args.no_autotune, graph=not args.no_graph, single=not args.no_extra,
+                       two_streams=not args.no_two_streams)
     tuned = head.get("autotune")
     plain = None
     if not args.no_extra:
