@@ -14,9 +14,9 @@
 // (twiddles, the store's S_i, the mask rows w, the key rows y, the last-digit
 // constants), so all of them are Shoup products a*w - floor(a*w'/2^64)*q with the
 // precomputed companion w' = floor(w*2^64/q), with a three-multiply quotient estimate.
-// Residues stay lazy, in [0, 4q) through the inverse transforms and the state and in
-// [0, 8q) through the forward transforms, and are made canonical only where the
-// reference's bytes depend on it (the digits and the outputs). The inverse transform carries no n^-1: S_i, w and y hold it already.
+// Residues stay lazy, in [0, 8q) through the inverse transforms and the state, in
+// [0, 16q) through the forward transforms and [0, 4q) in the online sum, and are made
+// canonical only where the reference's bytes depend on it (the digits and the outputs). The inverse transform carries no n^-1: S_i, w and y hold it already.
 //
 // Layout: thread t owns slots 4t..4t+3. The first inverse pass (bits 0-1) and the
 // last forward pass (bits 1-0) work on exactly those slots, so the recursion state
@@ -38,45 +38,75 @@ __device__ __forceinline__ uint64_t shoup(uint64_t a, uint64_t w, uint64_t wp) {
 __device__ __forceinline__ uint64_t shoup(uint64_t a, const Pair &p) { return shoup(a, p.w, p.wp); }
 
 constexpr uint64_t kFourQ = 4 * kQ;
-constexpr uint64_t kEightQ = 8 * kQ;  // < 2^63; 16q < 2^64 bounds the pointwise sums
+constexpr uint64_t kEightQ = 8 * kQ;  // < 2^63; 16q = 2^64 - 2^18 + 16 < 2^64 bounds every sum
+__device__ __forceinline__ uint64_t csub8q(uint64_t x) { return x >= kEightQ ? x - kEightQ : x; }
 __device__ __forceinline__ uint64_t csub4q(uint64_t x) { return x >= kFourQ ? x - kFourQ : x; }
 __device__ __forceinline__ uint64_t csub2q(uint64_t x) { return x >= kTwoQ ? x - kTwoQ : x; }
 __device__ __forceinline__ uint64_t csubq(uint64_t x) { return x >= kQ ? x - kQ : x; }
 // [0, 4q) -> canonical
 __device__ __forceinline__ uint64_t canon4(uint64_t x) { return csubq(csub2q(x)); }
+// [0, 8q) -> canonical
+__device__ __forceinline__ uint64_t canon8(uint64_t x) { return csubq(csub2q(csub4q(x))); }
 
-// Gentleman-Sande butterfly: inputs and outputs in [0, 4q).
+// Lazy ranges. Shoup products land in [0, 4q) for any 64-bit operand, so only the operand
+// that is added or subtracted needs a bound, and a reduction is spent only where a later sum
+// could reach 16q:
+//   inverse (Gentleman-Sande) values are in [0, 8q);
+//   forward (Cooley-Tukey) values are in [0, 16q), and a radix-4 pass reduces its two inputs
+//   that enter the first layer's sums to [0, 8q).
+//
+// Gentleman-Sande butterfly, x, y in [0, 8q): x <- (x + y) reduced to [0, 8q), y <- (x - y) w.
 __device__ __forceinline__ void gs(uint64_t &x, uint64_t &y, const Pair &w) {
-    uint64_t s = csub4q(x + y);
+    uint64_t s = csub8q(x + y);   // x + y < 16q
+    uint64_t d = x + kEightQ - y;  // (0, 16q)
+    x = s;
+    y = shoup(d, w);
+}
+// The same for x, y in [0, 4q) (two Shoup outputs): the sum stays below 8q unreduced.
+__device__ __forceinline__ void gs_small(uint64_t &x, uint64_t &y, const Pair &w) {
+    uint64_t s = x + y;           // [0, 8q)
     uint64_t d = x + kFourQ - y;  // (0, 8q)
     x = s;
     y = shoup(d, w);
 }
-// Cooley-Tukey butterfly (Harvey's lazy form): inputs and outputs in [0, 8q).
+// Cooley-Tukey butterfly (Harvey's form without the reduction of x): x in [0, 12q), y any
+// 64-bit value; outputs x + t and x + 4q - t with t = y w in [0, 4q), both below 16q.
 __device__ __forceinline__ void ct(uint64_t &x, uint64_t &y, const Pair &w) {
     uint64_t t = shoup(y, w);  // [0, 4q)
-    uint64_t u = csub4q(x);    // [0, 4q)
+    uint64_t u = x;
     x = u + t;
     y = u + kFourQ - t;
 }
 
-// Radix-4 inverse pass over bits (b, b+1) on a group's registers v[k] = x[base + (k << b)].
+// Radix-4 inverse pass over bits (b, b+1) on a group's registers v[k] = x[base + (k << b)],
+// inputs in [0, 8q). The first layer's differences are Shoup outputs in [0, 4q), so the
+// second layer's (v1, v3) butterfly needs no reduction. Outputs: v0, v1 in [0, 8q), v2, v3
+// in [0, 4q).
+__device__ __forceinline__ void gs4r(uint64_t v[4], const Pair &w0, const Pair &w1, const Pair &w2) {
+    gs(v[0], v[1], w0);
+    gs(v[2], v[3], w1);
+    gs(v[0], v[2], w2);
+    gs_small(v[1], v[3], w2);
+}
 template <int LOGN>
 __device__ __forceinline__ void gs4(uint64_t v[4], int b, int hi, const Pair *__restrict__ tw) {
     constexpr int N = 1 << LOGN;
     const int m0 = N >> (b + 1), m1 = N >> (b + 2);
     const Pair w0 = tw[m0 + 2 * hi], w1 = tw[m0 + 2 * hi + 1], w2 = tw[m1 + hi];
-    gs(v[0], v[1], w0);
-    gs(v[2], v[3], w1);
-    gs(v[0], v[2], w2);
-    gs(v[1], v[3], w2);
+    gs4r(v, w0, w1, w2);
 }
-// Radix-4 forward pass over bits (b+1, b).
-template <int LOGN>
+// Radix-4 forward pass over bits (b+1, b), inputs in [0, 16q) (SMALL: v0, v1 already below
+// 8q). v0 and v1 are the x operands of the first layer: reduced to [0, 8q), the first layer
+// leaves every value below 12q and the second below 16q.
+template <int LOGN, bool SMALL = false>
 __device__ __forceinline__ void ct4(uint64_t v[4], int b, int hi, const Pair *__restrict__ tw) {
     constexpr int N = 1 << LOGN;
     const int m0 = N >> (b + 2);
     const Pair w0 = tw[m0 + hi], w1 = tw[2 * m0 + 2 * hi], w2 = tw[2 * m0 + 2 * hi + 1];
+    if constexpr (!SMALL) {
+        v[0] = csub8q(v[0]);
+        v[1] = csub8q(v[1]);
+    }
     ct(v[0], v[2], w0);
     ct(v[1], v[3], w0);
     ct(v[0], v[1], w1);
@@ -147,10 +177,7 @@ __device__ __forceinline__ void inverse_to(uint64_t s[4], uint64_t *x, const Pai
     if constexpr (BL <= 2) warm = *warm_src;
     uint64_t v[4] = {s[0], s[1], s[2], s[3]};
     // the radix-4 inverse pass over bits 0-1 (gs4 at b = 0)
-    gs(v[0], v[1], w0);
-    gs(v[2], v[3], w1);
-    gs(v[0], v[2], w2);
-    gs(v[1], v[3], w2);
+    gs4r(v, w0, w1, w2);
     const int base = 4 * t;
 #pragma unroll
     for (int k = 0; k < 4; k++) x[lds_ix(base + k)] = v[k];
@@ -166,10 +193,7 @@ __device__ __forceinline__ void inverse_to(uint64_t s[4], uint64_t *x, const Pai
         uint64_t y[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) y[k] = x[lds_ix(gb + (k << 2))];
-        gs(y[0], y[1], u0);
-        gs(y[2], y[3], u1);
-        gs(y[0], y[2], u2);
-        gs(y[1], y[3], u2);
+        gs4r(y, u0, u1, u2);
 #pragma unroll
         for (int k = 0; k < 4; k++) x[lds_ix(gb + (k << 2))] = y[k];
         step_sync(6 <= 8 && 4 < STOP);
@@ -196,9 +220,9 @@ __device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi
         for (int r = 0; r < 2; r++) {
             const int e = t + r * (N / 4);
             uint64_t x = C[lds_ix(e)], y = C[lds_ix(e + N / 2)];
-            gs(x, y, twi[1]);
+            gs(x, y, twi[1]);  // x in [0, 8q), y a Shoup output in [0, 4q)
             uint64_t a0, a1, b0, b1;
-            gadget_digits<EXACT>(canon4(x), a0, a1);
+            gadget_digits<EXACT>(canon8(x), a0, a1);
             gadget_digits<EXACT>(canon4(y), b0, b1);
             ct(a0, b0, twf[1]);
             ct(a1, b1, twf[1]);
@@ -224,10 +248,13 @@ __device__ __forceinline__ void middle(uint64_t *C, uint64_t *D, const Pair *twi
 #pragma unroll
         for (int k = 0; k < 4; k++) v[k] = C[lds_ix(base + (k << b))];
         gs4<LOGN>(v, b, hi, twi);
-#pragma unroll
-        for (int k = 0; k < 4; k++) gadget_digits<EXACT>(canon4(v[k]), u[k], w[k]);
-        ct4<LOGN>(u, b, hi, twf);
-        ct4<LOGN>(w, b, hi, twf);
+        // gs4 leaves v0, v1 in [0, 8q) and v2, v3 in [0, 4q)
+        gadget_digits<EXACT>(canon8(v[0]), u[0], w[0]);
+        gadget_digits<EXACT>(canon8(v[1]), u[1], w[1]);
+        gadget_digits<EXACT>(canon4(v[2]), u[2], w[2]);
+        gadget_digits<EXACT>(canon4(v[3]), u[3], w[3]);
+        ct4<LOGN, true>(u, b, hi, twf);  // digits: below 2^21
+        ct4<LOGN, true>(w, b, hi, twf);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             C[lds_ix(base + (k << b))] = u[k];
@@ -438,16 +465,26 @@ void k_chain(CtxDev c, BatchDev bt) {
         inverse_to<LOGN, (LOGN & 1) ? LOGN - 1 : LOGN - 2, BL>(X, C, twi, twi_l,
                          reinterpret_cast<const uint32_t *>(S + 4 * tl), warm, tl);
         middle<LOGN, EXACT, BL>(C, D, twi_l, twf_l, twf, tl);
-        // last forward pass (bits 1-0) on own slots, then the pointwise work
-        uint64_t t0[4], t1[4];
+        // last forward pass (bits 1-0) on own slots, written back to those slots: the pointwise
+        // work reads each slot's two digit transforms from LDS when it needs them, which keeps
+        // 16 registers free for its row loads (no other lane touches these slots until the next
+        // step's first pass)
         const int base = 4 * tl;
+        {
+            uint64_t t0[4], t1[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            t0[k] = C[lds_ix(base + k)];
-            t1[k] = D[lds_ix(base + k)];
+            for (int k = 0; k < 4; k++) {
+                t0[k] = C[lds_ix(base + k)];
+                t1[k] = D[lds_ix(base + k)];
+            }
+            ct4<LOGN>(t0, 0, tl, twf);
+            ct4<LOGN>(t1, 0, tl, twf);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                C[lds_ix(base + k)] = t0[k];
+                D[lds_ix(base + k)] = t1[k];
+            }
         }
-        ct4<LOGN>(t0, 0, tl, twf);
-        ct4<LOGN>(t1, 0, tl, twf);
         asm volatile("" ::"v"(warm));  // the warm-up load completes here, long after its issue
         const uint32_t t = c.gpow[i];
 #pragma unroll
@@ -457,7 +494,7 @@ void k_chain(CtxDev c, BatchDev bt) {
             // exact gadget: the third digit's transform is kx*R + k0*T0 + k1*T1, folded into the
             // rows (W_2 = kx*w_2, W_0 = w_0 + k0*w_2, W_1 = w_1 + k1*w_2; same for y), so its
             // operand here is R itself
-            const uint64_t d[3] = {t0[k], t1[k], X[k]};
+            const uint64_t d[3] = {C[lds_ix(j)], D[lds_ix(j)], X[k]};
             uint64_t r = shoup(A[aix(src)], S[j]);
             uint64_t o = acc[k];
 #pragma unroll
@@ -465,11 +502,10 @@ void k_chain(CtxDev c, BatchDev bt) {
                 r += shoup(d[rr], c.wprime[(size_t)rr * N + src]);
                 o += shoup(d[rr], Y[(size_t)rr * N + src]);
             }
-            // r, o < (1 + ROWS) * 4q <= 16q < 2^64: back to [0, 4q)
-            r = r >= kEightQ ? r - kEightQ : r;
-            o = o >= kEightQ ? o - kEightQ : o;
-            X[k] = csub4q(r);
-            acc[k] = csub4q(o);
+            // r < (1 + ROWS) 4q <= 16q < 2^64: the state back to [0, 8q); o < acc + ROWS 4q
+            // <= 16q: the online sum back to [0, 4q), so the next step's o stays below 16q
+            X[k] = csub8q(r);
+            acc[k] = csub4q(csub8q(o));
         }
     }
 
@@ -500,20 +536,20 @@ void k_chain(CtxDev c, BatchDev bt) {
             for (int r = 0; r < 2; r++) {
                 const int e = tid + r * (N / 4);
                 uint64_t x = C[lds_ix(e)], y = C[lds_ix(e + N / 2)];
-                gs(x, y, twi_l[1]);
-                C[lds_ix(e)] = canon4(x);
+                gs(x, y, twi_l[1]);  // x in [0, 8q), y in [0, 4q)
+                C[lds_ix(e)] = canon8(x);
                 C[lds_ix(e + N / 2)] = canon4(y);
                 x = D[lds_ix(e)], y = D[lds_ix(e + N / 2)];
                 gs(x, y, twi_l[1]);
-                D[lds_ix(e)] = canon4(x);
+                D[lds_ix(e)] = canon8(x);
                 D[lds_ix(e + N / 2)] = canon4(y);
             }
         } else {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int e = tid + r * T;
-                C[lds_ix(e)] = canon4(C[lds_ix(e)]);
-                D[lds_ix(e)] = canon4(D[lds_ix(e)]);
+                C[lds_ix(e)] = canon8(C[lds_ix(e)]);  // inverse outputs: [0, 8q)
+                D[lds_ix(e)] = canon8(D[lds_ix(e)]);
             }
         }
         __syncthreads();
