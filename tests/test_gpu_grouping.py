@@ -129,3 +129,59 @@ def test_past_the_grouping_limit(gpu):
             ws.close()
         assert np.array_equal(outs[0][0][:count - 1], outs[1][0]) and np.array_equal(outs[0][1][:count - 1], outs[1][1])
         seat.close()
+
+
+@pytest.mark.parametrize("gamma", [256, 512, 1024])
+def test_wide_shards_runs_of_every_length(gpu, gamma):
+    """Wide shards (gamma >= 256, one 512 to 2048-byte row per wave of the grouped scan) are
+    scanned in tasks of up to two queries of one shard. Runs of 1 to 9 queries per shard cover
+    odd and even runs and their cuts into tasks; one shard's column edited after its store was
+    built is refused for exactly its queries. At gamma 256 every query is compared with the
+    ungrouped no-alloc path (which serves up to gamma 256), and at every width queries from runs
+    of 1, 4, 5 and 8 with the oracle."""
+    import torch
+
+    rng = np.random.default_rng(200 + gamma)
+    w_seed = bytes(range(40, 72))
+    shards = 10
+    cols = [rng.integers(0, 1 << 16, (gamma, N), dtype=np.uint64) for _ in range(shards)]
+    y = rng.integers(0, Q, (3, N), dtype=np.uint64)
+    with gpu.Context(gpu.params_d2048(), w_seed, gamma) as ctx:
+        handles = [ctx.upload_columns(i, cols[i]) for i in range(shards)]
+        for h in handles:
+            h.precompute_packing()
+        handles[9].write_value(1000, gamma - 1, int(cols[9][gamma - 1, 1000]) ^ 3)
+        seat = ctx.seat(y)
+        ids = np.concatenate([np.full(k, k, dtype=np.int64) for k in range(1, 10)] + [np.full(3, 9)])
+        ids = ids[np.random.default_rng(gamma).permutation(len(ids))]
+        edited = ids == 9
+        print(f"grouping: gamma {gamma}, runs of 1-9 queries per shard, {len(ids)} queries", flush=True)
+        if gamma == 256:
+            ((ga, gb, gs), (ua, ub, us)), _, _ = _run_both(gpu, torch, ctx, seat, ids, gamma, 300, oracle.Q36)
+            assert (us[edited] == gamma).all() and (us[~edited] == 0).all()
+            assert np.array_equal(gs, us)
+            assert np.array_equal(ga[~edited], ua[~edited]) and np.array_equal(gb[~edited], ub[~edited])
+        dev = torch.device("cuda:0")
+        count = len(ids)
+        seeds = rng.integers(0, 256, (count, 32), dtype=np.uint8)
+        bs = rng.integers(0, Q, (count, N), dtype=np.uint64)
+        bs[:, 5] = 0  # a zero b word: its reversed word is zero too (brev[n - 5])
+        ws = gpu.Workspace(ctx, count)
+        a = torch.zeros((count, N), dtype=torch.int64, device=dev)
+        b = torch.zeros((count, gamma), dtype=torch.int64, device=dev)
+        d_ids = torch.from_numpy(ids.astype(np.int32)).to(dev)
+        d_seeds = torch.from_numpy(seeds).to(dev)
+        d_b = torch.from_numpy(bs.view(np.int64)).to(dev)
+        ctx.respond_batch_device(seat, ws, d_ids.data_ptr(), d_seeds.data_ptr(), d_b.data_ptr(), count, a.data_ptr(),
+                                 b.data_ptr())
+        torch.cuda.synchronize()
+        status = ws.status(count)
+        ws.close()
+        assert (status[edited] == gamma).all() and (status[~edited] == 0).all()
+        ga, gb = a.cpu().numpy().view(np.uint64), b.cpu().numpy().view(np.uint64)
+        p = oracle.Pack(N, gamma)
+        for run in (1, 4, 5, 8):
+            k = int(np.flatnonzero(ids == run)[-1])
+            wa, wb = p.respond(w_seed, oracle.expand_seed(seeds[k].tobytes(), N), bs[k], cols[ids[k]], y)
+            assert np.array_equal(ga[k], wa) and np.array_equal(gb[k], wb[:gamma]), (run, k)
+        seat.close()
