@@ -273,7 +273,7 @@ typedef struct inspire_sm_seat inspire_sm_seat;
 typedef struct {
     uint32_t ring_dim;          /* 2048 served; the gfx950 kernels run 16..8192 */
     uint32_t gamma;             /* 512 served */
-    uint64_t q;                 /* INSPIRE_SM_Q served; at most 2^32 / 6 here (lazy residues) */
+    uint64_t q;                 /* INSPIRE_SM_Q served; any prime below 2^31 that validate accepts */
     uint64_t special_prime;     /* INSPIRE_SM_P served */
     uint32_t cell_bytes;        /* gamma / 2 */
     uint32_t record_bytes;      /* 246 served */

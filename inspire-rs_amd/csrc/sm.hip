@@ -26,7 +26,9 @@
 // the seat) are Montgomery products a*b*2^-32 with the rows stored as b = x*2^32, so
 // they need no companion words in HBM or registers. Both land in [0, 2m).
 // Residues mod q stay lazy in [0, 4q) (6q < 2^32); residues mod p (p < 2^31) stay canonical,
-// except the forward transforms' outputs, in [0, 2p).
+// except the forward transforms' outputs, in [0, 2p). A ciphertext prime above 2^32 / 6 (the
+// reference accepts any below 2^31) runs the kernels' WQ instantiation, whose residues mod q
+// follow the rules of p: canonical, forward transform outputs in [0, 2q).
 //
 // Layout: 8 slots per thread. Radix-8 passes over bits 0-2 (the thread's own slots
 // 8t..8t+7), 3-5, 6-8 ..., the top pass taking the remaining 1-3 bits; each round trip
@@ -256,31 +258,41 @@ __device__ __forceinline__ uint32_t lift_pq(uint32_t r, const SmCtxDev &c) {
     return x + (r > c.half_p ? c.neg_off : 0u);             // [0, 3q)
 }
 
+// The same lift for a ciphertext prime above 2^32 / 6, canonical: r mod q, less p mod q when
+// r > p/2 (neg_off = (q - p mod q) mod q).
+__device__ __forceinline__ uint32_t lift_pq_wide(uint32_t r, const SmCtxDev &c) {
+    const uint32_t x = red(r - __umulhi(r, c.q_barrett) * c.q, c.q);  // r mod q
+    return r > c.half_p ? red(x + c.neg_off, c.q) : x;
+}
+
 // iNTT of the own-slot registers v, centred lift, NTT: QP maps q -> p, else p -> q.
-// Leaves v in the other prime's transform domain (mod p in [0, 2p), mod q in [0, 4q)).
-template <int LOGN, bool QP, class Mid>
+// Leaves v in the other prime's transform domain (mod p in [0, 2p), mod q in [0, 4q), or
+// [0, 2q) under WQ). Inputs mod q: [0, 2q), canonical under WQ.
+template <int LOGN, bool QP, bool WQ, class Mid>
 __device__ __forceinline__ void round_trip(uint32_t v[8], uint32_t *X, const SmCtxDev &c, const Pair32 *twi,
                                            const Pair32 *twf, int t, Mid &&mid) {
     constexpr int BT = Plan<LOGN>::BT, RT = Plan<LOGN>::RT;
+    // canonical-residue transforms: always mod p, mod q under WQ
+    constexpr bool PI = QP ? WQ : true, PF = QP ? true : WQ;
     const uint32_t mi = QP ? c.q : c.p, mf = QP ? c.p : c.q;
-    inv_pass<LOGN, 0, 3, !QP>(v, t, twi, mi);
+    inv_pass<LOGN, 0, 3, PI>(v, t, twi, mi);
     own_store(X, v, t);
     pass_sync<NextUp<LOGN, 0>::value>();
-    inv_mid<LOGN, 3, !QP>(X, twi, mi, t);
+    inv_mid<LOGN, 3, PI>(X, twi, mi, t);
     {
         uint32_t u[8];
         lds_load<LOGN, BT, RT>(X, u, t);
-        inv_pass<LOGN, BT, RT, !QP>(u, t, twi, mi);
+        inv_pass<LOGN, BT, RT, PI>(u, t, twi, mi);
 #pragma unroll
-        for (int k = 0; k < 8; k++) u[k] = QP ? lift_qp(u[k], c) : lift_pq(u[k], c);
-        fwd_pass<LOGN, BT, RT, QP>(u, t, twf, mf);
+        for (int k = 0; k < 8; k++) u[k] = QP ? lift_qp(u[k], c) : WQ ? lift_pq_wide(u[k], c) : lift_pq(u[k], c);
+        fwd_pass<LOGN, BT, RT, PF>(u, t, twf, mf);
         lds_store<LOGN, BT, RT>(X, u, t);
     }
     pass_sync<LOGN>();
     mid();  // issued here: loads the caller needs after the transform
-    fwd_mid<LOGN, BT - 3, QP>(X, twf, mf, t);
+    fwd_mid<LOGN, BT - 3, PF>(X, twf, mf, t);
     own_load(X, v, t);
-    fwd_pass<LOGN, 0, 3, QP>(v, t, twf, mf);
+    fwd_pass<LOGN, 0, 3, PF>(v, t, twf, mf);
 }
 
 // Forward transform of the coefficients in X; returns the own slots in v.
@@ -337,7 +349,7 @@ struct SmOccupancy {
     // 2048 (n/8 <= 256 threads), and at 4096 and 8192 (512 and 1024 threads per workgroup)
     static constexpr int blocks = LOGN <= 11 ? 4 : LOGN == 12 ? 2 : 1;
 };
-template <int LOGN>
+template <int LOGN, bool WQ>
 __global__ __launch_bounds__((1 << LOGN) / 8, SmOccupancy<LOGN>::blocks) void k_sm_respond(SmCtxDev c, SmBatchDev bt) {
     constexpr int N = 1 << LOGN, T = N / 8, NW = Lds<LOGN>::words;
     // three buffers (27 KiB): the transform buffer X also serves the rotation of V (PV), the
@@ -374,18 +386,18 @@ __global__ __launch_bounds__((1 << LOGN) / 8, SmOccupancy<LOGN>::blocks) void k_
             __syncthreads();
         }
     }
-    // NTT(a), lazy in [0, 4q), parked in the query's slice of the mask output: like the
-    // seat's rows, it is read once per step, just ahead of its use, instead of pinning
-    // registers for all steps
+    // NTT(a), lazy in [0, 4q) ([0, 2q) under WQ), parked in the query's slice of the mask
+    // output: like the seat's rows, it is read once per step, just ahead of its use, instead of
+    // pinning registers for all steps. V: [0, 2q), canonical under WQ.
     uint32_t *apark = bt.mask_out + (size_t)qi * N + 8 * t;
     uint32_t V[8], BQ[8], BP[8];
     {
         uint32_t A[8], S[8];
-        forward_from_lds<LOGN, false>(X, A, c.twf_q, q, t);
+        forward_from_lds<LOGN, WQ>(X, A, c.twf_q, q, t);
         gstore8(apark, A);
         gload8(job.step1 + (size_t)(c.gamma - 1) * N + 8 * t, S);
 #pragma unroll
-        for (int k = 0; k < 8; k++) V[k] = red(mont(A[k], S[k], q, c.qninv), 2 * q);
+        for (int k = 0; k < 8; k++) V[k] = red(mont(A[k], S[k], q, c.qninv), WQ ? q : 2 * q);
     }
     const uint32_t *seat = job.seat + 8 * t;
 
@@ -422,13 +434,13 @@ __global__ __launch_bounds__((1 << LOGN) / 8, SmOccupancy<LOGN>::blocks) void k_
         }
         __syncthreads();  // every gather from PV is done before the transform reuses it
         // tp = NTT_p(lift(iNTT_q(Y)))
-        uint32_t v[8], S[8], A[8], WQ[8], KQ[8], WP[8], KP[8];
+        uint32_t v[8], S[8], A[8], WQR[8], KQ[8], WP[8], KP[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) v[k] = V[k];
         // the seat's rows are loaded after the transform, the table row (streamed from HBM)
         // during it: rows loaded early stay live across the transform, and at the 128-register
         // budget of four workgroups per CU each one adds spills
-        round_trip<LOGN, true>(v, X, c, twi_q, twf_p, t, [&] {});
+        round_trip<LOGN, true, WQ>(v, X, c, twi_q, twf_p, t, [&] {});
         {
             const uint32_t *sp = seat + opaque_s(0);
             gload8(sp + 2 * N, WP);
@@ -442,7 +454,7 @@ __global__ __launch_bounds__((1 << LOGN) / 8, SmOccupancy<LOGN>::blocks) void k_
             v[k] = red(mont(v[k], WP[k], p, c.pninv), p);
         }
         // r = NTT_q(lift(iNTT_p(tp * w_p)))
-        round_trip<LOGN, false>(v, X, c, twi_p, twf_q, t, [&] {
+        round_trip<LOGN, false, WQ>(v, X, c, twi_p, twf_q, t, [&] {
             own_store(PP, bp, t);  // every thread's gather from PP is behind the transform's barriers
             const int z2 = opaque_s(0);
             gload8(job.step1 + (size_t)i * N + 8 * t + z2, S);
@@ -450,21 +462,28 @@ __global__ __launch_bounds__((1 << LOGN) / 8, SmOccupancy<LOGN>::blocks) void k_
         {
             const int z2 = opaque_s(0);
             gload8(apark + z2, A);
-            gload8(seat + z2, WQ);
+            gload8(seat + z2, WQR);
             gload8(seat + N + z2, KQ);
         }
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             const uint32_t t1 = mont(A[k], S[k], q, c.qninv);
-            const uint32_t t2 = mont(V[k], WQ[k], q, c.qninv);
-            const uint32_t t3 = shoup(t2 + 4 * q - v[k], c.pinv_n, q);  // (0, 6q): 6q < 2^32
+            const uint32_t t2 = mont(V[k], WQR[k], q, c.qninv);
             const int r = (int)((opaque_v(rot[k >> 1]) >> (16 * (k & 1))) & 0xffffu);
-            bq[k] = red(PQ[r] + mont(V[k], KQ[k], q, c.qninv), 2 * q);
-            V[k] = red(t1 + t3, 2 * q);
+            if constexpr (WQ) {
+                // every sum of two residues below q: canonical operands, sums below 2q < 2^32
+                const uint32_t t3 = shoup(red(t2, q) + q - red(v[k], q), c.pinv_n, q);  // (0, 2q)
+                bq[k] = red(PQ[r], q) + red(mont(V[k], KQ[k], q, c.qninv), q);       // [0, 2q)
+                V[k] = red(red(t1, q) + red(t3, q), q);
+            } else {
+                const uint32_t t3 = shoup(t2 + 4 * q - v[k], c.pinv_n, q);  // (0, 6q): 6q < 2^32
+                bq[k] = red(PQ[r] + mont(V[k], KQ[k], q, c.qninv), 2 * q);
+                V[k] = red(t1 + t3, 2 * q);
+            }
         }
     }
 #pragma unroll
-    for (int k = 0; k < 8; k++) BQ[k] = bq[k];
+    for (int k = 0; k < 8; k++) BQ[k] = WQ ? red(bq[k], q) : bq[k];  // the WQ inverse takes canonical words
     own_load(PP, BP, t);  // stored by this thread in the last step
     __syncthreads();
 
@@ -513,8 +532,8 @@ __global__ __launch_bounds__((1 << LOGN) / 8, SmOccupancy<LOGN>::blocks) void k_
     }
 
     // --- mask = iNTT_q(V_0); body = mod_down(iNTT_q(B_q), iNTT_p(B_p)) + fold
-    inverse_to_lds<LOGN, false>(V, X, c.twi_q, q, t);
-    inverse_to_lds<LOGN, false>(BQ, PQ, c.twi_q, q, t);
+    inverse_to_lds<LOGN, WQ>(V, X, c.twi_q, q, t);
+    inverse_to_lds<LOGN, WQ>(BQ, PQ, c.twi_q, q, t);
     inverse_to_lds<LOGN, true>(BP, PP, c.twi_p, p, t);
     if (!bt.wire_out) {
         uint32_t *mask = bt.mask_out + (size_t)qi * N;
@@ -526,7 +545,8 @@ __global__ __launch_bounds__((1 << LOGN) / 8, SmOccupancy<LOGN>::blocks) void k_
             if (k < c.nb) {
                 const uint32_t xq = red(PQ[swz(k)], q);
                 const uint32_t xp = PP[swz(k)];
-                const uint32_t d = xq + 3 * q - lift_pq(xp, c);  // (0, 4q)
+                // (0, 4q); (0, 2q) under WQ
+                const uint32_t d = WQ ? xq + q - lift_pq_wide(xp, c) : xq + 3 * q - lift_pq(xp, c);
                 bout[k] = red(red(shoup(d, c.pinv, q), q) + F[j], q);
             }
         }
@@ -541,7 +561,7 @@ __global__ __launch_bounds__((1 << LOGN) / 8, SmOccupancy<LOGN>::blocks) void k_
         if (k < c.nb) {
             const uint32_t xq = red(PQ[swz(k)], q);
             const uint32_t xp = PP[swz(k)];
-            const uint32_t d = xq + 3 * q - lift_pq(xp, c);  // (0, 4q)
+            const uint32_t d = WQ ? xq + q - lift_pq_wide(xp, c) : xq + 3 * q - lift_pq(xp, c);
             PQ[swz(k)] = red(red(shoup(d, c.pinv, q), q) + F[j], q);
         }
     }
@@ -586,7 +606,7 @@ __global__ void k_sm_columns(const uint8_t *cells, int n, int cell_bytes, int8_t
 
 // S'_i = (n gamma)^-1 NTT_q(P_i), P_i = sum_j X^(j * g^(n-i)) h_j (step1_row_impl,
 // server.rs:330-383, whose tau_{g^i} gather this frame drops). One row i per workgroup.
-template <int LOGN>
+template <int LOGN, bool WQ>
 __global__ __launch_bounds__((1 << LOGN) / 8) void k_sm_table(SmCtxDev c, const int8_t *cols, uint32_t *step1) {
     constexpr int N = 1 << LOGN, T = N / 8;
     __shared__ __align__(16) uint32_t X[Lds<LOGN>::words];
@@ -617,7 +637,7 @@ __global__ __launch_bounds__((1 << LOGN) / 8) void k_sm_table(SmCtxDev c, const 
     for (int k = 0; k < 8; k++) X[swz(t + k * T)] = acc[k] < 0 ? (uint32_t)(acc[k] + (int32_t)c.q) : (uint32_t)acc[k];
     __syncthreads();
     uint32_t v[8];
-    forward_from_lds<LOGN, false>(X, v, c.twf_q, c.q, t);
+    forward_from_lds<LOGN, WQ>(X, v, c.twf_q, c.q, t);
 #pragma unroll
     for (int k = 0; k < 8; k++) v[k] = red(shoup(v[k], c.table_scale, c.q), c.q);
     gstore8(step1 + (size_t)i * N + 8 * t, v);
@@ -628,7 +648,7 @@ __global__ __launch_bounds__((1 << LOGN) / 8) void k_sm_table(SmCtxDev c, const 
 // Seat of one key (server.rs:177-203): the key mask from ChaCha20(seed) on stream
 // "key mask", q limb then p limb, each draw rejected in the short last interval
 // (mod.rs:216-237, 313-330); both key limbs; all in transform form as Montgomery operands.
-template <int LOGN>
+template <int LOGN, bool WQ>
 __global__ __launch_bounds__((1 << LOGN) / 8) void k_sm_seat(SmCtxDev c, const uint8_t *mask_seed,
                                                             const uint32_t *key_q, const uint32_t *key_p,
                                                             uint32_t *seat) {
@@ -677,11 +697,11 @@ __global__ __launch_bounds__((1 << LOGN) / 8) void k_sm_seat(SmCtxDev c, const u
         __syncthreads();
     }
     uint32_t v[8];
-    forward_from_lds<LOGN, false>(XQ, v, c.twf_q, q, t);  // w_q
+    forward_from_lds<LOGN, WQ>(XQ, v, c.twf_q, q, t);  // w_q
 #pragma unroll
     for (int k = 0; k < 8; k++) v[k] = red(shoup(v[k], c.seat_wq, q), q);
     gstore8(seat + 8 * t, v);
-    forward_from_lds<LOGN, false>(KQ, v, c.twf_q, q, t);  // key_q
+    forward_from_lds<LOGN, WQ>(KQ, v, c.twf_q, q, t);  // key_q
 #pragma unroll
     for (int k = 0; k < 8; k++) v[k] = red(shoup(v[k], c.seat_kq, q), q);
     gstore8(seat + N + 8 * t, v);
@@ -698,18 +718,27 @@ __global__ __launch_bounds__((1 << LOGN) / 8) void k_sm_seat(SmCtxDev c, const u
 template <int LOGN>
 hipError_t table_t(const SmCtxDev &c, const uint8_t *cells, int8_t *cols, uint32_t *step1, hipStream_t s) {
     hipLaunchKernelGGL(k_sm_columns, dim3(c.n), dim3(256), 0, s, cells, c.n, c.cell_bytes, cols);
-    hipLaunchKernelGGL(k_sm_table<LOGN>, dim3(c.gamma), dim3((1 << LOGN) / 8), 0, s, c, cols, step1);
+    if (c.wide_q)
+        hipLaunchKernelGGL((k_sm_table<LOGN, true>), dim3(c.gamma), dim3((1 << LOGN) / 8), 0, s, c, cols, step1);
+    else
+        hipLaunchKernelGGL((k_sm_table<LOGN, false>), dim3(c.gamma), dim3((1 << LOGN) / 8), 0, s, c, cols, step1);
     return hipGetLastError();
 }
 template <int LOGN>
 hipError_t seat_t(const SmCtxDev &c, const uint8_t *seed, const uint32_t *kq, const uint32_t *kp, uint32_t *seat,
                   hipStream_t s) {
-    hipLaunchKernelGGL(k_sm_seat<LOGN>, dim3(1), dim3((1 << LOGN) / 8), 0, s, c, seed, kq, kp, seat);
+    if (c.wide_q)
+        hipLaunchKernelGGL((k_sm_seat<LOGN, true>), dim3(1), dim3((1 << LOGN) / 8), 0, s, c, seed, kq, kp, seat);
+    else
+        hipLaunchKernelGGL((k_sm_seat<LOGN, false>), dim3(1), dim3((1 << LOGN) / 8), 0, s, c, seed, kq, kp, seat);
     return hipGetLastError();
 }
 template <int LOGN>
 hipError_t respond_t(const SmCtxDev &c, const SmBatchDev &b, hipStream_t s) {
-    hipLaunchKernelGGL(k_sm_respond<LOGN>, dim3(b.count), dim3((1 << LOGN) / 8), 0, s, c, b);
+    if (c.wide_q)
+        hipLaunchKernelGGL((k_sm_respond<LOGN, true>), dim3(b.count), dim3((1 << LOGN) / 8), 0, s, c, b);
+    else
+        hipLaunchKernelGGL((k_sm_respond<LOGN, false>), dim3(b.count), dim3((1 << LOGN) / 8), 0, s, c, b);
     return hipGetLastError();
 }
 

@@ -12,6 +12,11 @@ struct Pair32 {
     uint32_t w, wp;
 };
 
+// Largest ciphertext prime whose lazy residues fit a word: the recursion's widest lazy sum mod
+// q, t2 + 4q - v, is below 6q. Larger primes (the reference accepts any below 2^31) run the
+// kernels with canonical residues mod q.
+constexpr uint64_t kSmLazyQMax = 0xffffffffULL / 6;
+
 struct SmCtxDev {
     int logn, n, gamma, nb, cell_bytes;
     uint32_t q, p;
@@ -21,6 +26,7 @@ struct SmCtxDev {
     uint32_t q_barrett;   // floor(2^32 / q)
     uint32_t neg_off;     // (q - p mod q) mod q
     uint32_t g;           // 2n / gamma + 1, generator of the rotation subgroup
+    int wide_q;           // q > kSmLazyQMax: the kernels keep residues mod q canonical
     uint64_t q_recip, p_recip;  // floor(2^64 / m), reducing uniform u64 draws
     uint64_t zone_q, zone_p;    // (2^64 - 1) / m * m: draws at or above are rejected
     uint32_t qninv, pninv;  // -m^-1 mod 2^32 (Montgomery products, R = 2^32)

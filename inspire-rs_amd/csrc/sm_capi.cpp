@@ -160,8 +160,8 @@ int inspire_sm_server_create(const inspire_sm_params *params, inspire_sm_server 
     int rc = inspire_sm_params_validate(params, err, err_cap);
     if (rc) return rc;
     const uint64_t n = params->ring_dim, gamma = params->gamma, q = params->q, p = params->special_prime;
-    // what the gfx950 kernels run: n/8 threads per query with 8 slots each, lazy
-    // residues mod q in [0, 4q), and 4-byte words of nibbles
+    // what the gfx950 kernels run: n/8 threads per query with 8 slots each (rings up to
+    // 8192 in LDS) and 4-byte words of nibbles
     if (n > 8192) {
         sm_err(err, err_cap, "ring dimension %llu is outside the [16, 8192] the gfx950 kernels run",
                (unsigned long long)n);
@@ -169,12 +169,6 @@ int inspire_sm_server_create(const inspire_sm_params *params, inspire_sm_server 
     }
     if (gamma < 8) {
         sm_err(err, err_cap, "packed width %llu is below the 8 the gfx950 kernels run", (unsigned long long)gamma);
-        return INSPIRE_ERR_INVALID;
-    }
-    // the widest lazy sum mod q, t2 + 4q - v in the recursion step, is below 6q
-    if (q > 0xffffffffULL / 6) {
-        sm_err(err, err_cap, "ciphertext prime %llu is above the %llu (2^32 / 6) the gfx950 kernels' lazy residues fit in a word",
-               (unsigned long long)q, 0xffffffffULL / 6);
         return INSPIRE_ERR_INVALID;
     }
     int logn = 0;
@@ -227,6 +221,7 @@ int inspire_sm_server_create(const inspire_sm_params *params, inspire_sm_server 
     c.q_barrett = (uint32_t)((1ULL << 32) / q);
     c.neg_off = (uint32_t)((q - p % q) % q);
     c.g = (uint32_t)(2 * n / gamma + 1);
+    c.wide_q = q > kSmLazyQMax ? 1 : 0;  // every prime below 2^31 that Params::validate accepts
     c.q_recip = UINT64_MAX / q;
     c.p_recip = UINT64_MAX / p;
     c.zone_q = (UINT64_MAX / q) * q;

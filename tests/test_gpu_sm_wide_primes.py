@@ -1,14 +1,19 @@
 """The special-modulus kernels at ciphertext primes above the served one.
 
-The server accepts any prime q <= 2^32 / 6 (715,827,882) that Params::validate accepts
-(src/special_modulus/mod.rs:100-165): the recursion keeps residues mod q lazy below 6q. The
-served q (536,813,569) sits just under 2^29; these cases run the kernels at
+The server accepts every prime q below 2^31 that Params::validate accepts
+(src/special_modulus/mod.rs:100-165). Up to 2^32 / 6 (715,827,882) the recursion keeps residues
+mod q lazy below 6q; above it the kernels' WQ instantiation keeps them canonical, as it does mod
+p. The served q (536,813,569) sits just under 2^29; these cases run the kernels at
 
 * 536,903,681, the first NTT-friendly prime (1 mod 4096) above 2^29, where a product of a
   centred body coefficient (up to q/2) and a nibble of -8 no longer fits 32 bits, so the body
   fold (server.rs:398-410, i64 sums) must widen before it multiplies;
-* 715,825,153, the largest prime 1 mod 4096 the server accepts, where the lazy sums of the
-  recursion come within 17,000 of 2^32;
+* 715,825,153, the largest prime 1 mod 4096 of the lazy instantiation, where the lazy sums of
+  the recursion come within 17,000 of 2^32;
+* 716,259,329, the first prime 1 mod 2^14 above 2^32 / 6, the smallest of the canonical
+  instantiation;
+* 2,147,352,577, the largest prime 1 mod 2^14 below the served special prime 2,147,389,441 (and
+  below 2^31), where sums of two residues come within 2^22 of 2^32;
 
 at ring 2048 (gamma 512, the served cells) and at ring 64, with every nibble -8 (the largest
 fold) and with random cells, against the oracle (i64 and i128 arithmetic, oracle/sm_arith.h),
@@ -22,7 +27,7 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 SM_P = oracle.SM_P
-WIDE = [536_903_681, 715_825_153]
+WIDE = [536_903_681, 715_825_153, 716_259_329, 2_147_352_577]
 
 
 def _params(gpu, d, gamma, q):

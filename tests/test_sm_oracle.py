@@ -315,11 +315,39 @@ def test_invalid_sets_are_refused():
     p.q, p.special_prime = p.special_prime, p.q
     with pytest.raises(g.PirError, match="not above"):
         p.validate()
-    # a set the reference accepts but the kernels' lazy sums mod q (below 6q) do not fit in a
-    # word: 715,829,249 is prime, 1 mod 4096 and below p; refused at server creation, before
-    # any device call
+    # a ring the reference accepts but the kernels do not run (primes 1 mod 2^15): refused at
+    # server creation, before any device call
+    p = g.SmParams(16384, 512, 536_641_537, 2_147_352_577, 256, 256, 6.4, 2)
+    p.validate()
+    with pytest.raises(g.PirError, match=r"outside the \[16, 8192\] the gfx950 kernels run"):
+        g.SmServer(p)
+    # ciphertext primes above 2^32 / 6 up to 2^31 are accepted (the kernels keep residues mod q
+    # canonical there); primes at or above 2^31 are not (Params::validate)
     p = g.ring2048_cell256()
     p.q = 715_829_249
     p.validate()
-    with pytest.raises(g.PirError, match=r"above the 715827882 \(2\^32 / 6\)"):
-        g.SmServer(p)
+    p.q = 2_147_352_577
+    p.validate()
+    p.q, p.special_prime = 2_147_352_577, 2_147_483_659
+    with pytest.raises(g.PirError, match="not a prime below 2"):
+        p.validate()
+
+
+@pytest.mark.parametrize("q", [716_259_329, 2_147_352_577])
+def test_oracle_at_primes_above_the_lazy_bound_decodes(q):
+    # src/special_modulus/mod.rs:124-133 accepts ciphertext primes up to 2^31; the oracle's
+    # u64/i128 arithmetic follows the reference's at every one of them: responses to queries at
+    # the first and last rows, over random cells and over cells whose every nibble is -8 (the
+    # largest fold), decode to their records through the client
+    d, gamma = 64, 8
+    cell = record = gamma // 2
+    ring = oracle.SmRing(d=d, gamma=gamma, q=q, cell_bytes=cell, record_bytes=record)
+    worst = np.full(d * cell, 0x88, dtype=np.uint8)
+    for cells in (oracle.sm_kat_cells(q % 997, d, cell_bytes=cell, record_bytes=record), worst):
+        client = oracle.SmClient(ring, bytes([q % 251] * 32))
+        seed, kq, kp = client.packing_key()
+        table, seat = ring.table(cells.tobytes()), ring.seat(kq, kp, seed)
+        for row in (0, d - 1):
+            qs, body = client.query(row)
+            mask, out = ring.respond(table, seat, qs, body)
+            assert client.decode(mask, out) == bytes(cells[row * cell:(row + 1) * cell])
