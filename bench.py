@@ -656,6 +656,8 @@ def batched_leg(g, torch, dev, stream, ctx, seat, shards, B, M, launches, warmup
     if tune:
         # launch shape picked by timing the three shapes on synthetic queries at this launch size
         out["autotune"] = g.autotune(ctx, seat, ws, L)
+        # and the DB scan's row splits (its tile), at the same launch size
+        out["autotune"].update(g.autotune_scan(ctx, seat, ws, L))
         log(f"bench: autotune at {L} queries/launch over {shards} shards: {out['autotune']}")
     for i in range(warmup):
         issue(i)

@@ -72,6 +72,10 @@ uint32_t inspire_ctx_key_rows(const inspire_ctx *ctx);
  * runs a kernel built for three per CU (fewer registers). inspire_ctx_autotune picks
  * one by timing. */
 int inspire_ctx_set_queries_per_cu(inspire_ctx *ctx, uint32_t queries_per_cu, char *err, size_t err_cap);
+/* Row blocks per shard of the DB scan (no reference counterpart; results are unchanged):
+ * 8, 16 (default) or 32 at ring 2048 and up, 1 below; inspire_ctx_autotune_scan picks one
+ * by timing. The next launch of every workspace and serving slot of the context uses it. */
+int inspire_ctx_set_scan_splits(inspire_ctx *ctx, uint32_t splits, char *err, size_t err_cap);
 
 /* Upload one shard of the EncodedDatabase (ShardData, src/pir/setup.rs:280-289).
  * columns: num_columns x ring_dim column polynomials (ShardData::polynomials), each
@@ -225,6 +229,16 @@ void inspire_graph_destroy(inspire_graph *graph);
  * context, and reports the three times in kernel_ms and the choice in *chosen. */
 int inspire_ctx_autotune(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws, size_t count,
                          float kernel_ms[3], uint32_t *chosen, char *err, size_t err_cap);
+/* DB-scan tile autotuner (no reference counterpart; results are unchanged): at ring 2048
+ * and up, times the grouping and scan kernels on synthetic queries (count queries at
+ * uniform shard ids over the context's stored shards) with the scan split into 8, 16 and
+ * 32 row blocks per shard, keeps the fastest in the context (16 until tuned; the next
+ * launch of every workspace and serving slot uses it), and reports the three times in
+ * scan_ms and the choice in *chosen. Below ring 2048 the scan has one split: *chosen = 1,
+ * scan_ms zeros. Fewer splits suit launches that name many shards, more suit small
+ * launches (more workgroups per shard). */
+int inspire_ctx_autotune_scan(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws, size_t count,
+                              float scan_ms[3], uint32_t *chosen, char *err, size_t err_cap);
 
 /* Continuous-batching server over one context (the serving side of
  * respond_seeded_batch with registered keys, src/pir/respond.rs:1189, and of

@@ -350,6 +350,25 @@ int inspire_ctx_set_queries_per_cu(inspire_ctx *ctx, uint32_t queries_per_cu, ch
     return INSPIRE_OK;
 }
 
+int inspire_ctx_set_scan_splits(inspire_ctx *ctx, uint32_t splits, char *err, size_t err_cap) {
+    if (!ctx) {
+        set_err(err, err_cap, "null context");
+        return INSPIRE_ERR_INVALID;
+    }
+    if (ctx->n < 2048) {
+        if (splits == 1) return INSPIRE_OK;
+        set_err(err, err_cap, "the DB scan below ring 2048 has one split");
+        return INSPIRE_ERR_INVALID;
+    }
+    for (int k : kScanSplitChoices)
+        if ((uint32_t)k == splits) {
+            ctx->scan_splits.store(k, std::memory_order_relaxed);
+            return INSPIRE_OK;
+        }
+    set_err(err, err_cap, "scan splits must be 8, 16 or 32 at ring %d", ctx->n);
+    return INSPIRE_ERR_INVALID;
+}
+
 int inspire_shard_upload_columns(inspire_ctx *ctx, uint32_t shard_id, const uint64_t *columns, uint32_t num_columns,
                                  inspire_shard **out, char *err, size_t err_cap) {
     if (!ctx || !columns || !out) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
@@ -537,7 +556,7 @@ static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_wo
                           const uint32_t *d_shard_ids, const uint8_t *d_seeds, const uint64_t *d_b, size_t count,
                           uint64_t switch_modulus, uint64_t *d_a_out, uint64_t *d_b_prefix_out, void *stream,
                           float *timing, char *err, size_t err_cap, uint8_t *d_wire = nullptr,
-                          size_t wire_stride = 0, int queries_per_cu = 0) {
+                          size_t wire_stride = 0, int queries_per_cu = 0, int scan_splits = 0) {
     if (!ctx || !ws || ws->ctx != ctx) { set_err(err, err_cap, "workspace belongs to another context"); return INSPIRE_ERR_INVALID; }
     int rc = check_batch(ctx, seat, nullptr, count, err, err_cap);
     if (rc) return rc;
@@ -559,7 +578,7 @@ static int respond_device(inspire_ctx *ctx, const inspire_seat *seat, inspire_wo
     b.b = d_b;
     b.yprime = seat->yprime.as<Pair>();
     b.scan_part = ws->scan_part.as<uint64_t>();
-    b.ks = ctx->ks();
+    b.ks = scan_splits ? scan_splits : ctx->ks();  // a trial tile (inspire_ctx_autotune_scan)
     b.a_out = d_a_out;
     b.b_out = d_b_prefix_out;
     b.status = ws->status.as<int32_t>();
@@ -649,7 +668,8 @@ int inspire_respond_batch_device_noalloc(inspire_ctx *ctx, const inspire_seat *s
                 (unsigned long long)switch_modulus);
         return INSPIRE_ERR_INVALID;
     }
-    const size_t per_query = (size_t)ctx->ks() * ctx->gamma * 4 * sizeof(uint64_t);
+    const int ks = ctx->ks();  // read once: the limit and the launch agree if a tuning runs meanwhile
+    const size_t per_query = (size_t)ks * ctx->gamma * 4 * sizeof(uint64_t);
     if (count > INSPIRE_NOALLOC_SCRATCH_BYTES / per_query) {
         set_err(err, err_cap,
                 "batch of %zu queries needs %zu bytes of scan scratch; the allocation-free entry point holds %u "
@@ -688,7 +708,7 @@ int inspire_respond_batch_device_noalloc(inspire_ctx *ctx, const inspire_seat *s
     b.b = d_b;
     b.yprime = seat->yprime.as<Pair>();
     b.scan_part = static_cast<uint64_t *>(scr);
-    b.ks = ctx->ks();
+    b.ks = ks;
     b.a_out = d_a_out;
     b.b_out = d_b_prefix_out;
     b.status = d_status;
@@ -837,17 +857,15 @@ int inspire_graph_launch(inspire_graph *g, void *stream, char *err, size_t err_c
 
 void inspire_graph_destroy(inspire_graph *g) { delete g; }
 
-int inspire_ctx_autotune(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws, size_t count,
-                         float kernel_ms[3], uint32_t *chosen, char *err, size_t err_cap) {
-    if (!ctx || !ws || ws->ctx != ctx || !kernel_ms || !chosen) {
-        set_err(err, err_cap, "null argument or workspace of another context");
-        return INSPIRE_ERR_INVALID;
-    }
-    if (count == 0 || count > ws->max_count) {
-        set_err(err, err_cap, "batch of %zu queries does not fit the workspace's %zu", count, ws->max_count);
-        return INSPIRE_ERR_INVALID;
-    }
+// Synthetic queries of the real shape for the autotuners: ids over the shards with a store
+// (round robin, or uniform draws), distinct seeds, b uniform-looking below q; device copies.
+struct TuneBatch {
+    DevBuf ids, seeds, b, a, bp;
+};
+static int tune_batch(inspire_ctx *ctx, size_t count, bool uniform, TuneBatch &tb, char *err, size_t err_cap) {
     std::vector<uint32_t> ids;
+    uint64_t x = 0x9e3779b97f4a7c15ULL;
+    auto next = [&x]() { x ^= x << 13; x ^= x >> 7; x ^= x << 17; return x; };
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
         std::vector<uint32_t> stored;
@@ -857,24 +875,44 @@ int inspire_ctx_autotune(inspire_ctx *ctx, const inspire_seat *seat, inspire_wor
             set_err(err, err_cap, "autotune needs a shard with its packing store built");
             return INSPIRE_ERR_INVALID;
         }
-        for (size_t k = 0; k < count; k++) ids.push_back(stored[k % stored.size()]);
+        for (size_t k = 0; k < count; k++) ids.push_back(stored[uniform ? next() % stored.size() : k % stored.size()]);
     }
-    // synthetic queries of the real shape: distinct seeds, b uniform-looking below q
     const size_t n = ctx->n, g = ctx->gamma;
     std::vector<uint8_t> seeds(count * 32);
     std::vector<uint64_t> b(count * n);
-    uint64_t x = 0x9e3779b97f4a7c15ULL;
-    for (auto &v : seeds) { x ^= x << 13; x ^= x >> 7; x ^= x << 17; v = (uint8_t)x; }
-    for (auto &v : b) { x ^= x << 13; x ^= x >> 7; x ^= x << 17; v = x % kQ; }
-    DevBuf d_ids, d_seeds, d_b, d_a, d_bp;
-    HIP_TRY(d_ids.alloc(count * 4));
-    HIP_TRY(d_seeds.alloc(count * 32));
-    HIP_TRY(d_b.alloc(count * n * 8));
-    HIP_TRY(d_a.alloc(count * n * 8));
-    HIP_TRY(d_bp.alloc(count * g * 8));
-    HIP_TRY(hipMemcpy(d_ids.p, ids.data(), count * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d_seeds.p, seeds.data(), count * 32, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d_b.p, b.data(), count * n * 8, hipMemcpyHostToDevice));
+    for (auto &v : seeds) v = (uint8_t)next();
+    for (auto &v : b) v = next() % kQ;
+    HIP_TRY(tb.ids.alloc(count * 4));
+    HIP_TRY(tb.seeds.alloc(count * 32));
+    HIP_TRY(tb.b.alloc(count * n * 8));
+    HIP_TRY(tb.a.alloc(count * n * 8));
+    HIP_TRY(tb.bp.alloc(count * g * 8));
+    HIP_TRY(hipMemcpy(tb.ids.p, ids.data(), count * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(tb.seeds.p, seeds.data(), count * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(tb.b.p, b.data(), count * n * 8, hipMemcpyHostToDevice));
+    return INSPIRE_OK;
+}
+
+static int tune_args(inspire_ctx *ctx, inspire_workspace *ws, size_t count, const void *out_a, const void *out_b,
+                     char *err, size_t err_cap) {
+    if (!ctx || !ws || ws->ctx != ctx || !out_a || !out_b) {
+        set_err(err, err_cap, "null argument or workspace of another context");
+        return INSPIRE_ERR_INVALID;
+    }
+    if (count == 0 || count > ws->max_count) {
+        set_err(err, err_cap, "batch of %zu queries does not fit the workspace's %zu", count, ws->max_count);
+        return INSPIRE_ERR_INVALID;
+    }
+    return INSPIRE_OK;
+}
+
+int inspire_ctx_autotune(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws, size_t count,
+                         float kernel_ms[3], uint32_t *chosen, char *err, size_t err_cap) {
+    int rc = tune_args(ctx, ws, count, kernel_ms, chosen, err, err_cap);
+    if (rc) return rc;
+    TuneBatch tb;
+    rc = tune_batch(ctx, count, false, tb, err, err_cap);
+    if (rc) return rc;
     // each shape is tried on its own launches; the context's shape changes only at the end,
     // so launches from other threads meanwhile keep the shape they had
     float best = 0;
@@ -882,9 +920,9 @@ int inspire_ctx_autotune(inspire_ctx *ctx, const inspire_seat *seat, inspire_wor
     for (int qpc = 1; qpc <= 3; qpc++) {
         float t[2] = {0, 0}, sum = 0;
         for (int rep = 0; rep < 3; rep++) {  // one warm-up, two timed
-            int rc = respond_device(ctx, seat, ws, d_ids.as<uint32_t>(), d_seeds.as<uint8_t>(), d_b.as<uint64_t>(),
-                                    count, 0, d_a.as<uint64_t>(), d_bp.as<uint64_t>(), nullptr, t, err, err_cap,
-                                    nullptr, 0, qpc);
+            rc = respond_device(ctx, seat, ws, tb.ids.as<uint32_t>(), tb.seeds.as<uint8_t>(), tb.b.as<uint64_t>(),
+                                count, 0, tb.a.as<uint64_t>(), tb.bp.as<uint64_t>(), nullptr, t, err, err_cap,
+                                nullptr, 0, qpc);
             if (rc) return rc;
             if (rep) sum += t[0] + t[1];
         }
@@ -896,6 +934,40 @@ int inspire_ctx_autotune(inspire_ctx *ctx, const inspire_seat *seat, inspire_wor
         ctx->queries_per_cu = (int)pick;
     }
     *chosen = pick;
+    return INSPIRE_OK;
+}
+
+int inspire_ctx_autotune_scan(inspire_ctx *ctx, const inspire_seat *seat, inspire_workspace *ws, size_t count,
+                              float scan_ms[3], uint32_t *chosen, char *err, size_t err_cap) {
+    int rc = tune_args(ctx, ws, count, scan_ms, chosen, err, err_cap);
+    if (rc) return rc;
+    if (ctx->n < 2048) {  // one split: nothing to choose
+        for (int k = 0; k < 3; k++) scan_ms[k] = 0;
+        *chosen = 1;
+        return INSPIRE_OK;
+    }
+    // shard ids drawn uniformly, as a launch's queries arrive: the tasks of the grouped scan
+    // (shards named by one, two or more queries) have the mix real launches have
+    TuneBatch tb;
+    rc = tune_batch(ctx, count, true, tb, err, err_cap);
+    if (rc) return rc;
+    float best = 0;
+    int pick = kScanSplitChoices[0];
+    for (int k = 0; k < 3; k++) {
+        const int ks = kScanSplitChoices[k];
+        float t[2] = {0, 0}, sum = 0;
+        for (int rep = 0; rep < 3; rep++) {  // one warm-up, two timed
+            rc = respond_device(ctx, seat, ws, tb.ids.as<uint32_t>(), tb.seeds.as<uint8_t>(), tb.b.as<uint64_t>(),
+                                count, 0, tb.a.as<uint64_t>(), tb.bp.as<uint64_t>(), nullptr, t, err, err_cap,
+                                nullptr, 0, 0, ks);
+            if (rc) return rc;
+            if (rep) sum += t[0];  // grouping and scan kernels
+        }
+        scan_ms[k] = sum / 2;
+        if (k == 0 || sum / 2 < best) { best = sum / 2; pick = ks; }
+    }
+    ctx->scan_splits.store(pick, std::memory_order_relaxed);
+    *chosen = (uint32_t)pick;
     return INSPIRE_OK;
 }
 

@@ -95,6 +95,12 @@ __host__ __device__ inline bool batch_grouped(const CtxDev &c, const BatchDev &b
     return b.order && b.first && b.ngroups && b.count <= kGroupMax && c.gamma >= 4;
 }
 
+// Row-split counts of the DB scan at ring 2048 and up (a launch's scan tile, picked by the
+// autotuner; 16 by default): every split is a multiple of 8 rows (n / 32 = 64 at n = 2048) and a
+// multiple of the four splits a wide shard's workgroup holds.
+constexpr int kScanSplitChoices[3] = {8, 16, 32};
+constexpr int kMaxScanSplits = 32;
+
 // Workgroup size of the DB scans over 4-column groups (gamma >= 4): one lane per (row split,
 // group), at most 256.
 __host__ __device__ inline int scan_threads(int gamma, int ks) {

@@ -172,7 +172,12 @@ struct inspire_ctx {
         for (void *p : retired) (void)hipFree(p);
     }
 
-    int ks() const { return n >= 2048 ? 16 : 1; }
+    // Row splits of the DB scan at ring 2048 and up: 16 unless inspire_ctx_autotune_scan picked
+    // another of kScanSplitChoices. Scan scratch is sized for kMaxScanSplits, so a change applies
+    // to the next launch of every workspace and serving slot; each launch reads it once.
+    std::atomic<int> scan_splits{16};
+    int ks() const { return n >= 2048 ? scan_splits.load(std::memory_order_relaxed) : 1; }
+    int ks_cap() const { return n >= 2048 ? kMaxScanSplits : 1; }
 
     CtxDev dev() const {
         CtxDev c;
@@ -226,7 +231,7 @@ inline int ws_reserve(inspire_workspace *ws, size_t count, char *err, size_t err
         return INSPIRE_ERR_INVALID;
     }
     size_t c = count < 1 ? 1 : count;
-    HIP_TRY(ws->scan_part.alloc(c * ctx->ks() * ctx->gamma * 4 * sizeof(uint64_t)));
+    HIP_TRY(ws->scan_part.alloc(c * ctx->ks_cap() * ctx->gamma * 4 * sizeof(uint64_t)));
     HIP_TRY(ws->status.alloc(c * sizeof(int32_t)));
     const size_t gc = c < (size_t)kGroupMax ? c : (size_t)kGroupMax;
     HIP_TRY(ws->group.alloc((2 * gc + 2) * sizeof(uint32_t)));

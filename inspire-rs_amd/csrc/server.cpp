@@ -120,7 +120,7 @@ int alloc_slot(inspire_server *sv, Slot &s, char *err, size_t err_cap) {
     HIP_TRY(s.d_seats.alloc(c * sizeof(void *)));
     HIP_TRY(s.d_a.alloc(c * n * 8));
     HIP_TRY(s.d_bp.alloc(c * g * 8));
-    HIP_TRY(s.scan_part.alloc(c * sv->ctx->ks() * g * 4 * 8));
+    HIP_TRY(s.scan_part.alloc(c * sv->ctx->ks_cap() * g * 4 * 8));
     HIP_TRY(s.status.alloc(c * 4));
     const size_t gc = c < (size_t)kGroupMax ? c : (size_t)kGroupMax;
     HIP_TRY(s.group.alloc((2 * gc + 2) * 4));

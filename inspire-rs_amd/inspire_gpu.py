@@ -107,6 +107,9 @@ def lib():
         L.inspire_graph_launch.argtypes = [vp, vp, cp, sz]
         L.inspire_graph_destroy.argtypes = [vp]
         L.inspire_ctx_autotune.argtypes = [vp, vp, vp, sz, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(u32), cp, sz]
+        L.inspire_ctx_set_scan_splits.argtypes = [vp, u32, cp, sz]
+        L.inspire_ctx_autotune_scan.argtypes = [vp, vp, vp, sz, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(u32),
+                                                cp, sz]
         smp = ctypes.POINTER(SmParams)
         L.inspire_sm_params_validate.argtypes = [smp, cp, sz]
         L.inspire_sm_server_create.argtypes = [smp, pp, cp, sz]
@@ -242,6 +245,10 @@ class Context:
 
     def set_queries_per_cu(self, k):
         _call(lib().inspire_ctx_set_queries_per_cu, self._h, k)
+
+    def set_scan_splits(self, k):
+        """Row blocks per shard of the DB scan: 8, 16 or 32 at ring 2048 and up."""
+        _call(lib().inspire_ctx_set_scan_splits, self._h, k)
 
     def close(self):
         for s in list(self.shards.values()):
@@ -394,6 +401,16 @@ def autotune(ctx, seat, ws, count):
     _call(lib().inspire_ctx_autotune, ctx._h, seat._h, ws._h, count, ms, ctypes.byref(chosen))
     return {"queries_per_cu": chosen.value, "ms_qpc1": round(ms[0], 4), "ms_qpc2": round(ms[1], 4),
             "ms_qpc3": round(ms[2], 4)}
+
+
+def autotune_scan(ctx, seat, ws, count):
+    """inspire_ctx_autotune_scan: times the DB scan split into 8, 16 and 32 row blocks per shard at
+    `count` queries (uniform shard ids), keeps the fastest."""
+    ms = (ctypes.c_float * 3)()
+    chosen = ctypes.c_uint32()
+    _call(lib().inspire_ctx_autotune_scan, ctx._h, seat._h, ws._h, count, ms, ctypes.byref(chosen))
+    return {"scan_splits": chosen.value, "scan_ms_ks8": round(ms[0], 4), "scan_ms_ks16": round(ms[1], 4),
+            "scan_ms_ks32": round(ms[2], 4)}
 
 
 class Server:

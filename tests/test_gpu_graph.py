@@ -127,3 +127,63 @@ def test_autotune_keeps_results(gpu):
         ws.close()
         seat.close()
         ctx.close()
+
+
+def test_scan_splits_keep_results_and_autotune_picks_one(gpu):
+    """The DB scan's row blocks per shard (8, 16, 32) change no byte: device launches at each, the
+    no-alloc path at 32 (its scratch holds fewer queries then) and a workspace created before the
+    change all equal the launch at 16; the scan autotuner keeps one of the three, and a query
+    after tuning equals the oracle."""
+    import torch
+
+    n, gamma, count = 2048, 256, 64
+    rng = np.random.default_rng(6)
+    dev = torch.device("cuda:0")
+    ctx, seat, cols, y, w_seed = _setup(gpu, rng)
+    ws = gpu.Workspace(ctx, count)
+    ids, seeds, bs = _batch(rng, count, len(cols), n)
+    d_ids = torch.from_numpy(ids).to(dev)
+    d_seeds = torch.from_numpy(seeds).to(dev)
+    d_b = torch.from_numpy(bs.view(np.int64)).to(dev)
+
+    def launch():
+        a = torch.zeros((count, n), dtype=torch.int64, device=dev)
+        b = torch.zeros((count, gamma), dtype=torch.int64, device=dev)
+        ctx.respond_batch_device(seat, ws, d_ids.data_ptr(), d_seeds.data_ptr(), d_b.data_ptr(), count,
+                                 a.data_ptr(), b.data_ptr())
+        torch.cuda.synchronize()
+        assert (ws.status(count) == 0).all()
+        return a.cpu().numpy(), b.cpu().numpy()
+
+    try:
+        want = launch()
+        for k in (8, 32, 16):
+            print(f"scan splits {k}", flush=True)
+            ctx.set_scan_splits(k)
+            got = launch()
+            assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]), k
+        ctx.set_scan_splits(32)
+        a = torch.zeros((count, n), dtype=torch.int64, device=dev)
+        b = torch.zeros((count, gamma), dtype=torch.int64, device=dev)
+        st = torch.zeros(count, dtype=torch.int32, device=dev)
+        gpu.respond_batch_device_noalloc(ctx, seat, d_ids.data_ptr(), d_seeds.data_ptr(), d_b.data_ptr(), count,
+                                         a.data_ptr(), b.data_ptr(), st.data_ptr())
+        torch.cuda.synchronize()
+        assert (st.cpu().numpy() == 0).all()
+        assert np.array_equal(a.cpu().numpy(), want[0]) and np.array_equal(b.cpu().numpy(), want[1])
+        with pytest.raises(gpu.PirError, match="8, 16 or 32"):
+            ctx.set_scan_splits(12)
+        r = gpu.autotune_scan(ctx, seat, ws, count)
+        print(f"scan autotune at {count} queries: {r}", flush=True)
+        assert r["scan_splits"] in (8, 16, 32)
+        assert min(r["scan_ms_ks8"], r["scan_ms_ks16"], r["scan_ms_ks32"]) > 0
+        got = launch()
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+        p = oracle.Pack(n, gamma)
+        k = 3
+        wa, wb = p.respond(w_seed, oracle.expand_seed(seeds[k].tobytes(), n), bs[k], cols[ids[k]], y)
+        assert np.array_equal(got[0][k].view(np.uint64), wa) and np.array_equal(got[1][k].view(np.uint64), wb[:gamma])
+    finally:
+        ws.close()
+        seat.close()
+        ctx.close()
