@@ -30,6 +30,20 @@ ENTRY = 512
 Q36 = 68718428161
 HBM_PEAK_GBS = 8000.0
 VALU_MAD_PEAK_T = 256 * 64 * 2.4e9 / 1e12  # 39.3 T v_mad_u64_u32 lane-ops/s
+# VALU instruction issue: one wave instruction (64 lanes on a 16-lane SIMD) per 4 cycles per SIMD
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4  # 614.4 G wave instructions/s
+
+
+def valu_issue(insts_per_launch, kernel_ms, counter_source):
+    """Fraction of peak VALU instruction issue of a kernel: its VALU wave instructions per launch
+    (rocprofv3 SQ_INSTS_VALU of the same workload, committed beside the profile; the count is fixed
+    by the launch's shape) over its measured kernel time."""
+    if insts_per_launch is None:
+        return None
+    rate = insts_per_launch / (kernel_ms * 1e-3)
+    return {"achieved": round(rate / 1e9, 2), "peak": round(VALU_ISSUE_PEAK / 1e9, 1), "unit": "G wave instructions/s",
+            "frac": round(rate / VALU_ISSUE_PEAK, 4), "insts_per_launch": round(insts_per_launch),
+            "source": counter_source}
 
 
 def log(msg):
@@ -507,11 +521,13 @@ def special_modulus(g, torch, dev, stream, args):
     bytes_per_query = SM_GAMMA * SM_N * 4 + SM_N * SM_CELL + 4 * SM_N * 4 + SM_N * 4 + SM_N * 4 + 2 * SM_RECORD * 4
     # HBM bytes per launch measured by rocprofv3 counters for this workload (2560 queries per
     # launch), committed beside the profile; scaled to this run's launch size
-    traffic = None
+    traffic = insts = None
     tpath = os.path.join(ROOT, "profiles", "sm_respond_batched2560_ring2048_cell256_traffic.json")
     if os.path.exists(tpath):
         tj = json.load(open(tpath))
         traffic = round(tj["hbm_bytes_per_dispatch"]["inspire::k_sm_respond<11>/2560"] * L / 2560)
+        insts = tj.get("valu_insts_per_dispatch", {}).get("inspire::k_sm_respond<11>/2560")
+        insts = None if insts is None else insts * L / 2560
     out = {
         "value": round(qps, 1), "unit": "queries/s",
         "workload": f"special_modulus ring2048_cell256: {len(tables)} step-1 tables ({len(tables) * SM_N} cells of "
@@ -534,7 +550,8 @@ def special_modulus(g, torch, dev, stream, args):
                      "timing": "HIP events on the launch stream around this kernel in each launch of the timed region",
                      "hbm_gbs": round(L * bytes_per_query / (kms * 1e-3) / 1e9, 1),
                      "definition": "3 v_mul_lo/hi_u32 per 32-bit modular product (Shoup or Montgomery), "
-                                   "algorithmic; peak = 256 CU x 64 lanes x 2.4 GHz"},
+                                   "algorithmic; peak = 256 CU x 64 lanes x 2.4 GHz",
+                     "valu_issue": valu_issue(insts, kms, "profiles/sm_respond_batched2560_ring2048_cell256.md")},
     }
     if not args.no_served:
         out["served"] = sm_served(g, srv, tables, rng, rate_qps=0.7 * qps, seconds=args.served_seconds,
@@ -945,12 +962,14 @@ def main():
 
     # HBM bytes per launch measured by rocprofv3 counters for this workload, committed beside the
     # profile; scaled to this run's launch size
-    traffic_chain = traffic_scan = None
+    traffic_chain = traffic_scan = chain_insts = None
     tpath = os.path.join(ROOT, "profiles", "respond_batched512_d2048_g256_2p20_traffic.json")
     if os.path.exists(tpath):
         tj = json.load(open(tpath))
         traffic_chain = round(tj["k_chain_512_queries_bytes"] * L / 512)
         traffic_scan = round(tj["k_scan_512_queries_bytes"] * L / 512)
+        if "k_chain_512_queries_valu_insts" in tj:
+            chain_insts = tj["k_chain_512_queries_valu_insts"] * L / 512
 
     sm = None
     if not args.no_sm:
@@ -1029,7 +1048,9 @@ def main():
                      "timing": "HIP events on the launch stream around this kernel in each launch of the timed region",
                      "modmul60_per_s": round(chain_gmms * 1e9, 0),
                      "definition": "4 v_mad_u64_u32 (32x32->64) per 60-bit modular product, algorithmic; peak = "
-                                   "256 CU x 64 lanes x 2.4 GHz (v_mad_u64_u32 issues at half the simple-op rate)"},
+                                   "256 CU x 64 lanes x 2.4 GHz (v_mad_u64_u32 issues at half the simple-op rate)",
+                     "valu_issue": valu_issue(chain_insts, chain_avg,
+                                              "profiles/respond_batched512_d2048_g256_2p20.md")},
         "roofline_scan": {"bound": "hbm", "kernel": "k_scan_grouped", "achieved": round(scan_gbs, 2),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(scan_gbs / HBM_PEAK_GBS, 4),
                           "traffic": traffic_scan, "launch_ms": round(scan_avg, 4),
