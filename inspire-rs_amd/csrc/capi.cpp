@@ -210,8 +210,9 @@ int inspire_ctx_create(const inspire_params *params, const uint8_t w_seed[32], u
     if (!params || !out) { set_err(err, err_cap, "null argument"); return INSPIRE_ERR_INVALID; }
     *out = nullptr;
     const uint32_t n = params->ring_dim;
-    if (n != 256 && n != 2048 && n != 4096) {
-        set_err(err, err_cap, "ring_dim %u has no gfx950 kernel here; built ring dimensions are 256, 2048 and 4096",
+    if (n != 256 && n != 512 && n != 1024 && n != 2048 && n != 4096) {
+        set_err(err, err_cap,
+                "ring_dim %u has no gfx950 kernel here; built ring dimensions are 256, 512, 1024, 2048 and 4096",
                 n);
         return INSPIRE_ERR_INVALID;
     }
@@ -238,7 +239,8 @@ int inspire_ctx_create(const inspire_params *params, const uint8_t w_seed[32], u
     inspire_ctx *ctx = new (std::nothrow) inspire_ctx();
     if (!ctx) { set_err(err, err_cap, "out of host memory"); return INSPIRE_ERR_DEVICE; }
     ctx->n = (int)n;
-    ctx->logn = n == 256 ? 8 : n == 2048 ? 11 : 12;
+    ctx->logn = 0;
+    while ((1u << ctx->logn) < n) ctx->logn++;
     ctx->gamma = (int)num_columns;
     ctx->dropped = params->approximate_packing_gadget ? 1 : 0;
     ctx->rows = 3 - ctx->dropped;

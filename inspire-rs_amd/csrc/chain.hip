@@ -619,6 +619,8 @@ hipError_t launch_chain_only(const CtxDev &c, const BatchDev &b, hipStream_t s) 
     if (b.count == 0) return hipSuccess;
     switch (c.logn) {
         case 8: return launch_chain_t<8>(c, b, s);
+        case 9: return launch_chain_t<9>(c, b, s);
+        case 10: return launch_chain_t<10>(c, b, s);
         case 11: return launch_chain_t<11>(c, b, s);
         case 12: return launch_chain_t<12>(c, b, s);
         default: return hipErrorInvalidValue;

@@ -499,6 +499,8 @@ hipError_t launch_expand_mask(int logn, int rows, const uint8_t *seed, uint64_t 
                               const uint64_t *psi, uint64_t inv_n, Pair *out, hipStream_t s) {
     switch (logn) {
         case 8: hipLaunchKernelGGL(k_expand_mask<8>, dim3(rows), dim3(kThreads), 0, s, seed, stream, dropped, psi, inv_n, out); break;
+        case 9: hipLaunchKernelGGL(k_expand_mask<9>, dim3(rows), dim3(kThreads), 0, s, seed, stream, dropped, psi, inv_n, out); break;
+        case 10: hipLaunchKernelGGL(k_expand_mask<10>, dim3(rows), dim3(kThreads), 0, s, seed, stream, dropped, psi, inv_n, out); break;
         case 11: hipLaunchKernelGGL(k_expand_mask<11>, dim3(rows), dim3(kThreads), 0, s, seed, stream, dropped, psi, inv_n, out); break;
         case 12: hipLaunchKernelGGL(k_expand_mask<12>, dim3(rows), dim3(kThreads), 0, s, seed, stream, dropped, psi, inv_n, out); break;
         default: return hipErrorInvalidValue;
@@ -510,6 +512,8 @@ hipError_t launch_seat(int logn, int rows, const uint64_t *y, const uint64_t *ps
                        hipStream_t s) {
     switch (logn) {
         case 8: hipLaunchKernelGGL(k_seat<8>, dim3(rows), dim3(kThreads), 0, s, y, psi, inv_n, out); break;
+        case 9: hipLaunchKernelGGL(k_seat<9>, dim3(rows), dim3(kThreads), 0, s, y, psi, inv_n, out); break;
+        case 10: hipLaunchKernelGGL(k_seat<10>, dim3(rows), dim3(kThreads), 0, s, y, psi, inv_n, out); break;
         case 11: hipLaunchKernelGGL(k_seat<11>, dim3(rows), dim3(kThreads), 0, s, y, psi, inv_n, out); break;
         case 12: hipLaunchKernelGGL(k_seat<12>, dim3(rows), dim3(kThreads), 0, s, y, psi, inv_n, out); break;
         default: return hipErrorInvalidValue;
@@ -524,6 +528,8 @@ hipError_t launch_store_build(const CtxDev &c, const uint16_t *records, uint16_t
                        col_scratch);
     switch (c.logn) {
         case 8: hipLaunchKernelGGL(k_store_build<8>, dim3(c.gamma), dim3(kThreads), 0, s, c, col_scratch, store); break;
+        case 9: hipLaunchKernelGGL(k_store_build<9>, dim3(c.gamma), dim3(kThreads), 0, s, c, col_scratch, store); break;
+        case 10: hipLaunchKernelGGL(k_store_build<10>, dim3(c.gamma), dim3(kThreads), 0, s, c, col_scratch, store); break;
         case 11: hipLaunchKernelGGL(k_store_build<11>, dim3(c.gamma), dim3(kThreads), 0, s, c, col_scratch, store); break;
         case 12: hipLaunchKernelGGL(k_store_build<12>, dim3(c.gamma), dim3(kThreads), 0, s, c, col_scratch, store); break;
         default: return hipErrorInvalidValue;

@@ -191,3 +191,25 @@ def test_extract_refusals(kat):
         oracle.parse_response(bytes(bad))
     with pytest.raises(ValueError):
         oracle.parse_response(bytes(wire[:-1]))
+
+
+@pytest.mark.parametrize("d", [512, 1024])
+@pytest.mark.parametrize("approximate", [False, True], ids=["exact", "approximate"])
+def test_cell_responses_decode_at_rings_512_and_1024(d, approximate):
+    """The same decode check over the oracle's responses at the rings 512 and 1024 the d2048 prime
+    and gadget admit, per query and through the shard store, unswitched and at 36 bits."""
+    entry = 32
+    c = oracle.Cell(d, entry, 0x5249_4e47 + d, approximate=approximate)
+    p = oracle.Pack(d, c.gamma, 3, 1 if approximate else 0)
+    for index in (0, d - 1, 2 * d + 5):
+        shard, seed, b = c.query(index)
+        cols = c.columns(shard)
+        a = oracle.expand_seed(seed, d)
+        wa, wb = p.respond(c.w_seed, a, b, cols, c.y_body)
+        sa, sb = p.respond(c.w_seed, a, b, cols, c.y_body, p.store_build(cols))
+        assert np.array_equal(wa, sa) and np.array_equal(wb, sb)
+        rec = c.db[index * entry:(index + 1) * entry].tobytes()
+        assert oracle.extract_inspiring(c.secret, wa, wb[:c.gamma], entry) == rec, index
+        got = oracle.extract_inspiring(c.secret, oracle.mod_switch(wa), oracle.mod_switch(wb[:c.gamma]), entry,
+                                       c.gamma, oracle.Q36)
+        assert got == rec, index

@@ -36,7 +36,7 @@ def test_chacha20_rfc7539_block():
     assert list(out) == want
 
 
-@pytest.mark.parametrize("n", [16, 256, 2048, 4096])
+@pytest.mark.parametrize("n", [16, 256, 512, 1024, 2048, 4096])
 def test_ntt_is_negacyclic_convolution(n):
     rng = np.random.default_rng(n)
     a = rng.integers(0, Q, n, dtype=np.uint64)
@@ -75,7 +75,7 @@ def test_montgomery_equals_plain_product():
         assert ntt.mont_mul(a, b) == a * b * rinv % Q
 
 
-@pytest.mark.parametrize("n,gamma", [(256, 16), (2048, 256), (4096, 256)])
+@pytest.mark.parametrize("n,gamma", [(256, 16), (512, 64), (1024, 128), (2048, 256), (4096, 256)])
 def test_automorph_tables_invariant_and_closed_form(n, gamma):
     p = oracle.Pack(n, gamma)
     ntt = oracle.Ntt(n)
@@ -135,11 +135,12 @@ def test_store_path_equals_per_query_path():
     assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
 
 
-def test_store_path_equals_per_query_path_at_ring_4096():
-    # InspireParams::d4096 (params.rs:392-436): the same invariant on the wider ring, for both
-    # gadgets
-    rng = np.random.default_rng(4096)
-    n, gamma = 4096, 16
+@pytest.mark.parametrize("n", [512, 1024, 4096])
+def test_store_path_equals_per_query_path_at_other_rings(n):
+    # InspireParams::d4096 (params.rs:392-436) and the rings 512 and 1024 the same prime admits
+    # (2n divides q - 1, params.rs:535-580): the same invariant on each ring, for both gadgets
+    rng = np.random.default_rng(n)
+    gamma = 16
     cols = rng.integers(0, 1 << 16, (gamma, n), dtype=np.uint64)
     b = rng.integers(0, Q, n, dtype=np.uint64)
     a = oracle.expand_seed(bytes(range(32)), n)
