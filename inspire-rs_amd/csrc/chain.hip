@@ -36,6 +36,10 @@ namespace inspire {
 // Every product here is shoup_lazy (modq.hpp): result in [0, 4q) for any a < 2^64.
 __device__ __forceinline__ uint64_t shoup(uint64_t a, uint64_t w, uint64_t wp) { return shoup_lazy(a, w, wp); }
 __device__ __forceinline__ uint64_t shoup(uint64_t a, const Pair &p) { return shoup(a, p.w, p.wp); }
+// store rows and key rows, read through the global address space (glob, device.hpp)
+__device__ __forceinline__ uint64_t shoup(uint64_t a, const __attribute__((address_space(1))) Pair &p) {
+    return shoup(a, p.w, p.wp);
+}
 
 constexpr uint64_t kFourQ = 4 * kQ;
 constexpr uint64_t kEightQ = 8 * kQ;  // < 2^63; 16q = 2^64 - 2^18 + 16 < 2^64 bounds every sum
@@ -171,7 +175,7 @@ __device__ __forceinline__ void lds_fwd_pass2(uint64_t *x, uint64_t *y, int b, c
 // places late (vector loads complete in order, so no vector load may come between).
 template <int LOGN, int STOP, int BL>
 __device__ __forceinline__ void inverse_to(uint64_t s[4], uint64_t *x, const Pair *tw, const Pair *tw_l,
-                                           const uint32_t *warm_src, uint32_t &warm, int t) {
+                                           Glob<uint32_t> warm_src, uint32_t &warm, int t) {
     constexpr int N = 1 << LOGN;
     const Pair w0 = tw[N / 2 + 2 * t], w1 = tw[N / 2 + 2 * t + 1], w2 = tw[N / 4 + t];
     if constexpr (BL <= 2) warm = *warm_src;
@@ -348,7 +352,7 @@ void k_chain(CtxDev c, BatchDev bt) {
         return;
     }
     const Pair *twf = c.twf, *twi = c.twi;
-    const Pair *Y = bt.yprimes ? bt.yprimes[qi] : bt.yprime;
+    const Glob<Pair> Y = glob(bt.yprimes ? bt.yprimes[qi] : bt.yprime);
     // Twiddles of the passes at base bit BL and up (ladder entries below n >> BL: 16 KiB at
     // n = 2048 for BL = 2, 4 KiB for BL = 4) live in LDS for the whole kernel; the lower
     // passes read theirs from L2.
@@ -440,7 +444,7 @@ void k_chain(CtxDev c, BatchDev bt) {
     uint64_t X[4], acc[4];
     {
         const uint32_t t = c.gpow[gamma - 1];
-        const Pair *S = sh.store + (size_t)(gamma - 1) * N;
+        const Glob<Pair> S = glob(sh.store + (size_t)(gamma - 1) * N);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int j = 4 * tid + k;
@@ -457,13 +461,13 @@ void k_chain(CtxDev c, BatchDev bt) {
         // registers when hoisted)
         int tl = tid;
         asm volatile("" : "+v"(tl));
-        const Pair *S = sh.store + (size_t)i * N;
+        const Glob<Pair> S = glob(sh.store + (size_t)i * N);
         // this step's store row is read at the end of the step; touching its lines now (one
         // word per thread, unconditionally, so the wait counts stay exact) lets the HBM fetch
         // overlap the transforms
         uint32_t warm = 0;
         inverse_to<LOGN, (LOGN & 1) ? LOGN - 1 : LOGN - 2, BL>(X, C, twi, twi_l,
-                         reinterpret_cast<const uint32_t *>(S + 4 * tl), warm, tl);
+                         reinterpret_cast<Glob<uint32_t>>(S + 4 * tl), warm, tl);
         middle<LOGN, EXACT, BL>(C, D, twi_l, twf_l, twf, tl);
         // last forward pass (bits 1-0) on own slots, written back to those slots: the pointwise
         // work reads each slot's two digit transforms from LDS when it needs them, which keeps

@@ -30,6 +30,15 @@ __device__ inline ShardDev shard_entry(const ShardDev *table, int num_shards, ui
     return sid < (uint32_t)num_shards ? table[sid] : z;
 }
 __device__ inline bool shard_ready(const ShardDev &s) { return s.records && s.store && s.tag_key && s.tags; }
+// Device memory through the global address space. A pointer read from memory (the shard table,
+// a seat array, a job entry) is generic to the compiler, and a generic (flat) load also counts
+// against the LDS and scalar-load counter, so every LDS or scalar wait behind it waits for HBM as
+// well, and a uniform generic load cannot become a scalar load. Casting such pointers keeps the
+// loads global.
+template <class T>
+using Glob = const __attribute__((address_space(1))) T *;
+template <class T>
+__device__ inline Glob<T> glob(const T *p) { return (Glob<T>)p; }
 // A pointer every lane holds alike, kept in scalar registers.
 template <class T>
 __device__ inline T *uniform(T *p) {

@@ -149,16 +149,22 @@ __device__ __forceinline__ void own_store(uint32_t *X, const uint32_t v[8], int 
     *reinterpret_cast<uint4 *>(X + swz(8 * t)) = make_uint4(v[0], v[1], v[2], v[3]);
     *reinterpret_cast<uint4 *>(X + swz(8 * t + 4)) = make_uint4(v[4], v[5], v[6], v[7]);
 }
+// Device-memory accesses through the global address space: pointers read from a job entry are
+// generic to the compiler, and a generic (flat) load also counts against the LDS counter, so
+// every LDS wait of the transform behind it would wait for HBM as well.
+typedef uint32_t U32x4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) U32x4 *GVec;
+typedef __attribute__((address_space(1))) U32x4 *GVecOut;
 __device__ __forceinline__ void gload8(const uint32_t *g, uint32_t v[8]) {
-    const uint4 *p = reinterpret_cast<const uint4 *>(g);
-    const uint4 a = p[0], b = p[1];
+    const GVec p = (GVec)g;
+    const U32x4 a = p[0], b = p[1];
     v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
     v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
 }
 __device__ __forceinline__ void gstore8(uint32_t *g, const uint32_t v[8]) {
-    uint4 *p = reinterpret_cast<uint4 *>(g);
-    p[0] = make_uint4(v[0], v[1], v[2], v[3]);
-    p[1] = make_uint4(v[4], v[5], v[6], v[7]);
+    const GVecOut p = (GVecOut)g;
+    p[0] = U32x4{v[0], v[1], v[2], v[3]};
+    p[1] = U32x4{v[4], v[5], v[6], v[7]};
 }
 
 // Inverse (Gentleman-Sande) stages B .. B+R-1 on the thread's groups. The twiddle of
